@@ -1,0 +1,283 @@
+/*
+ * tentaco.h - C ABI of the MI355X (gfx950) CSF MTTKRP / TTV library.
+ *
+ * Every function here has C linkage, takes plain pointers and sizes, and
+ * reports failure through its return code plus tentaco_last_error(); no
+ * entry point calls exit(). The reference's boundary (gpu_kernels.cuh:1-73 of
+ * zhang677/tenTACO) is reproduced in three layers:
+ *
+ *  1. Compute-style entry points, host pointers in, A filled on return:
+ *     tentaco_mttkrp3/4/5, tentaco_ttv, tentaco_mttkrp (any order).
+ *  2. The reference's own symbol names with their void signatures
+ *     (mttkrp_gpu_taco, ttv_csf_gpu_taco, the 67 mttkrp_{R}_eb_{sr|pr}_...
+ *     tuning variants, get_gpu_timer_result), so a caller that was compiled
+ *     against gpu_kernels.cuh relinks against this library unchanged.
+ *  3. A device-resident API (upload a CSF once, run many MTTKRP/TTV calls on
+ *     device factors, on a caller stream) for CP-ALS-style loops, batching
+ *     and graph capture.
+ *
+ * Operand contract (matches gpu_library.h:780-859 / ds.h:368-436):
+ *   B  CSF tensor, order N in 2..5: indices[0] = {pos={0,n0}, crd[n0]},
+ *      indices[m] = {pos, crd} for m >= 1, vals = FP32[nnz]. int32 indices.
+ *   C, D, E, F  factor matrices for levels 1..N-1: dimensions = {rows, R},
+ *      row-major FP32, element [row*R + col] (ds.h:401).
+ *   A  MTTKRP output: dimensions = {rows, R}; every row is overwritten
+ *      (rows of empty slices become 0). Row of root node x is crd0[x].
+ *   A  TTV output: FP32 vector aligned with B's level N-2 positions (the
+ *      (i,j) fibers of a 3-mode tensor); its index pattern is B's levels
+ *      0..N-2, so only vals are written.
+ */
+#ifndef TENTACO_H
+#define TENTACO_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include "taco_tensor_t.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define TENTACO_OK 0
+#define TENTACO_EINVAL 1        /* malformed operand or argument */
+#define TENTACO_EHIP 2          /* HIP runtime error */
+#define TENTACO_ENOMEM 3        /* device or host allocation failed */
+#define TENTACO_EUNSUPPORTED 4  /* valid request this build does not handle */
+#define TENTACO_ENOGPU 5        /* no usable gfx950 device */
+
+/* Variant selector for the MTTKRP kernels (tentaco_variant_name lists them).
+ * TENTACO_VARIANT_AUTO lets the library choose (autotuned table or learned
+ * selector, falling back to the fixed default). */
+#define TENTACO_VARIANT_AUTO (-1)
+#define TENTACO_VARIANT_DEFAULT (-2)  /* the fixed TACO-schedule default */
+
+/* Message describing the last failure on the calling thread ("" if none). */
+const char* tentaco_last_error(void);
+/* Library version, major*10000 + minor*100 + patch. */
+int tentaco_version(void);
+/* 0 when a gfx950 device is usable, else TENTACO_ENOGPU (no kernel runs). */
+int tentaco_device_check(void);
+
+/* ---- layer 1: compute-style entry points (host pointers) --------------- */
+/* Replace mttkrp3_cpu_taco (tensor_kernels/mttkrp3_csf_cpu_taco.h:7) and the
+ * GPU wrappers of gpu_kernels.cuh:5-73 for a 3-mode CSF. Returns 0. */
+int tentaco_mttkrp3(taco_tensor_t* A, taco_tensor_t* B, taco_tensor_t* C,
+                    taco_tensor_t* D);
+/* Replaces mttkrp4_cpu_taco (tensor_kernels/mttkrp4_csf_cpu_taco.h:7). */
+int tentaco_mttkrp4(taco_tensor_t* A, taco_tensor_t* B, taco_tensor_t* C,
+                    taco_tensor_t* D, taco_tensor_t* E);
+/* Replaces mttkrp5_cpu_taco (tensor_kernels/mttkrp5_csf_cpu_taco.h:7). */
+int tentaco_mttkrp5(taco_tensor_t* A, taco_tensor_t* B, taco_tensor_t* C,
+                    taco_tensor_t* D, taco_tensor_t* E, taco_tensor_t* F);
+/* Any order 2..5: factors[m-1] is the factor of level m. variant as above.
+ * Same contract as tentaco_mttkrp3 (gpu_kernels.cuh:5). The order is passed
+ * explicitly because the reference's to_taco_tensor (ds.h:368-384) leaves
+ * taco_tensor_t.order unset. */
+int tentaco_mttkrp(int order, taco_tensor_t* A, taco_tensor_t* B,
+                   taco_tensor_t* const* factors, int variant);
+/* Replaces ttv_csf_cpu_taco (tensor_kernels/ttv_csf_cpu_taco.h:24) and
+ * ttv_csf_gpu_taco (gpu_kernels.cuh:6): A[f] = sum_k B(f,k) c(k) over the
+ * level N-2 positions f of B. */
+int tentaco_ttv(taco_tensor_t* A, taco_tensor_t* B, taco_tensor_t* c);
+/* TTV over the last mode of an order-2..5 CSF (order passed explicitly). */
+int tentaco_ttv_n(int order, taco_tensor_t* A, taco_tensor_t* B,
+                  taco_tensor_t* c);
+
+/* ---- layer 2: reference symbol names (gpu_kernels.cuh:4-73) ------------ */
+/* Milliseconds of the last call's hot region (partition + zero + kernel),
+ * hipEvent-timed like GPUTimer (gpu_library.h:949-988); -1 before any call. */
+float get_gpu_timer_result(void);
+/* TACO schedule (mttkrp3_csf_gpu_taco.h:73); runs the fixed default variant. */
+void mttkrp_gpu_taco(taco_tensor_t* A, taco_tensor_t* B, taco_tensor_t* C,
+                     taco_tensor_t* D);
+/* ttv_csf_gpu_taco.h:89. */
+void ttv_csf_gpu_taco(taco_tensor_t* A, taco_tensor_t* B, taco_tensor_t* C);
+
+/* Tuning variants of gpu_kernels.cuh:7-73. eb_sr_{TB}_{g}_{c} runs the
+ * edge-serial kernel at block TB with run length g (c, the columns per
+ * thread, is fixed by R in the wave64 lane layout). eb_pr_{TB}_{Nnz}_{gs}
+ * runs the edge-parallel kernel at block TB with reduction group gs. */
+#define TENTACO_DECL_VARIANT(name)                                             \
+  void name(taco_tensor_t* A, taco_tensor_t* B, taco_tensor_t* C,            \
+            taco_tensor_t* D);
+TENTACO_DECL_VARIANT(mttkrp_32_eb_sr_512_4_1)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_pr_256_64_32_manual)
+TENTACO_DECL_VARIANT(mttkrp_64_eb_pr_256_32_32)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_pr_256_32_4)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_pr_256_32_8)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_pr_256_32_16)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_pr_256_32_32)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_pr_256_64_4)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_pr_256_64_8)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_pr_256_64_16)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_pr_256_64_32)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_pr_256_128_4)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_pr_256_128_8)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_pr_256_128_16)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_pr_256_128_32)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_pr_256_256_4)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_pr_256_256_8)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_pr_256_256_16)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_pr_256_256_32)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_pr_512_64_4)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_pr_512_64_8)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_pr_512_64_16)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_pr_512_64_32)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_pr_512_128_4)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_pr_512_128_8)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_pr_512_128_16)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_pr_512_128_32)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_pr_512_256_4)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_pr_512_256_8)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_pr_512_256_16)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_pr_512_256_32)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_pr_512_512_4)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_pr_512_512_8)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_pr_512_512_16)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_pr_512_512_32)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_sr_256_4_1)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_sr_256_4_2)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_sr_256_4_4)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_sr_256_4_8)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_sr_256_8_1)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_sr_256_8_2)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_sr_256_8_4)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_sr_256_8_8)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_sr_256_16_1)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_sr_256_16_2)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_sr_256_16_4)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_sr_256_16_8)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_sr_256_32_1)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_sr_256_32_2)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_sr_256_32_4)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_sr_256_32_8)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_sr_512_4_2)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_sr_512_4_4)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_sr_512_4_8)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_sr_512_8_1)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_sr_512_8_2)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_sr_512_8_4)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_sr_512_8_8)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_sr_512_16_1)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_sr_512_16_2)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_sr_512_16_4)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_sr_512_16_8)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_sr_512_32_1)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_sr_512_32_2)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_sr_512_32_4)
+TENTACO_DECL_VARIANT(mttkrp_32_eb_sr_512_32_8)
+#undef TENTACO_DECL_VARIANT
+
+/* ---- variant table ------------------------------------------------------ */
+/* Number of MTTKRP kernel variants (the sweep/selector space). */
+int tentaco_num_variants(void);
+/* "es_b256_g16"-style name of variant v, or NULL when out of range. */
+const char* tentaco_variant_name(int v);
+/* Decode variant v: family (0 edge-serial, 1 edge-parallel, 2 slice-row),
+ * block size and group size. Returns 0 or TENTACO_EINVAL. */
+int tentaco_variant_params(int v, int* family, int* block, int* group);
+/* The fixed default variant (the TACO schedule eb_sr 512/g=4). */
+int tentaco_default_variant(void);
+
+/* ---- layer 3: device-resident API -------------------------------------- */
+typedef struct tentaco_csf tentaco_csf; /* opaque device-resident CSF */
+
+/* Validate a host CSF of the given order (contract above) and copy it to the
+ * device. The handle owns its device memory; free with tentaco_csf_free. Not
+ * tied to a stream. Validation rejects pos arrays that decrease or overrun,
+ * root coordinates that are not strictly increasing, and coordinates outside
+ * B->dimensions, so no kernel can read out of bounds. */
+int tentaco_csf_upload(const taco_tensor_t* B, int order, tentaco_csf** out);
+int tentaco_csf_free(tentaco_csf* h);
+/* order, nnz, number of root nodes, and node counts per level (nodes[5]). */
+int tentaco_csf_info(const tentaco_csf* h, int* order, int64_t* nnz,
+                     int32_t* nodes);
+/* Reserve the per-handle workspace a variant needs, so that later calls on
+ * this handle allocate nothing (required before graph capture). */
+int tentaco_csf_reserve(tentaco_csf* h, int R, int variant);
+
+/* MTTKRP on device buffers: factors_dev[m-1] (rows_m x R, row-major) for
+ * levels m = 1..N-1; A_dev has a_rows x R floats and is fully overwritten.
+ * Enqueued on `stream` (a hipStream_t; NULL = default stream); returns
+ * after enqueueing. */
+int tentaco_mttkrp_dev(tentaco_csf* h, int R, const float* const* factors_dev,
+                       float* A_dev, int32_t a_rows, int variant,
+                       void* stream);
+/* Autotuner (replaces codegen.py + the *_tune.sh sweeps of the reference):
+ * times every variant reps times on this handle, rank and device factors
+ * (hipEvents on `stream`, median per variant), records the fastest in the
+ * handle so TENTACO_VARIANT_AUTO uses it, and returns it in *best. times_ms
+ * (optional, tentaco_num_variants() entries) receives each median, -1 for a
+ * variant this rank cannot run. Allocates workspace; not for graph capture. */
+int tentaco_csf_autotune(tentaco_csf* h, int R, const float* const* factors_dev,
+                         float* A_dev, int32_t a_rows, int reps, void* stream,
+                         int* best, float* times_ms);
+/* Record a variant as this handle's choice for rank R (e.g. from a stored
+ * sweep or the learned selector). */
+int tentaco_csf_set_tuned(tentaco_csf* h, int R, int variant);
+
+/* TTV on device buffers: c_dev has the last mode's extent, A_dev has one
+ * float per level N-2 node (nodes[N-2]). */
+int tentaco_ttv_dev(tentaco_csf* h, const float* c_dev, float* A_dev,
+                    void* stream);
+
+/* Device memory helpers for hosts without their own HIP runtime binding. */
+int tentaco_dev_malloc(void** ptr, size_t bytes);
+int tentaco_dev_free(void* ptr);
+int tentaco_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int tentaco_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int tentaco_dev_synchronize(void);
+
+/* Overwrite a 512 MiB device buffer on `stream` so the next kernel starts with
+ * cold caches (GPU analog of Timer::clear_cache, timers.h:62-75). */
+int tentaco_flush_cache(void* stream);
+/* hipEvent timing of a device-API call sequence on `stream`: record start,
+ * record stop + synchronize, elapsed ms. */
+int tentaco_timer_start(void* stream);
+int tentaco_timer_stop(void* stream, float* ms);
+
+/* ---- host-side CSF construction ---------------------------------------- */
+/* An owned host CSF (dense root, int32 indices, FP32 values) with a
+ * taco_tensor_t view. Replaces the reference's splatt load path
+ * (read_tensor / to_taco_tensor / compress_top_level, ds.h:351-436). */
+typedef struct tentaco_host_csf tentaco_host_csf;
+
+/* Lexicographic COO -> CSF: coords is order x nnz (mode-major, coords[m*nnz +
+ * q]); duplicate coordinates are summed in input order (FP32); the root is
+ * dense (pos = {0, dims[0]}, crd = 0..dims[0]-1). */
+int tentaco_coo_to_csf(int order, const int32_t* dims, int64_t nnz,
+                       const int32_t* coords, const float* vals,
+                       tentaco_host_csf** out);
+
+/* Synthetic CSF generator (deterministic in seed, independent of thread
+ * count): root slice ids Zipf(slice_s) drawn without repetition until nnz is
+ * reached; level-1 fan-out Zipf(fanout_s) over dims[1]; levels 2..N-2
+ * fan-out round(lognormal(mid_mu, mid_sigma)); leaf fiber length
+ * round(lognormal(leaf_mu, leaf_sigma)) capped at leaf_cap; distinct sorted
+ * uniform children coordinates; values U[-1,1). The last fiber is truncated
+ * so the tensor holds exactly nnz nonzeros when reachable. */
+typedef struct tentaco_synth_params {
+  int order;
+  int32_t dims[5];
+  int64_t nnz;
+  double slice_s;
+  double fanout_s;
+  double mid_mu, mid_sigma;
+  double leaf_mu, leaf_sigma;
+  int32_t leaf_cap;
+  uint64_t seed;
+} tentaco_synth_params;
+int tentaco_synth_csf(const tentaco_synth_params* p, tentaco_host_csf** out);
+
+/* View valid until tentaco_host_csf_free; order field is set. */
+taco_tensor_t* tentaco_host_csf_view(tentaco_host_csf* h);
+/* Number of nodes at a level (level order-1 = nonzeros), -1 if invalid. */
+int64_t tentaco_host_csf_nodes(const tentaco_host_csf* h, int level);
+int tentaco_host_csf_free(tentaco_host_csf* h);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* TENTACO_H */

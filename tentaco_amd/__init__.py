@@ -1,0 +1,440 @@
+"""Python host mirror of the tentaco C ABI (include/tentaco.h).
+
+The product is ``libtentaco.so`` (HIP for gfx950, built in-tree by
+``tentaco_amd/Makefile``); this module only binds it with ctypes, so callers in
+Python see the same operator interface as C/C++ callers of the reference's
+``gpu_kernels.cuh``: ``mttkrp`` / ``ttv`` take a CSF tensor and factor
+matrices (host arrays) and return the output, ``DeviceCsf`` is the
+device-resident handle for repeated calls. There is no CPU fallback: if the
+library is missing or no gfx950 device is present, calls raise.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_double, c_float, c_int, c_int32, c_int64, c_size_t, c_uint64, c_void_p
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtentaco.so")
+
+VARIANT_AUTO = -1
+VARIANT_DEFAULT = -2
+
+OK, EINVAL, EHIP, ENOMEM, EUNSUPPORTED, ENOGPU = 0, 1, 2, 3, 4, 5
+
+
+class TentacoError(RuntimeError):
+    """A tentaco entry point returned a nonzero status."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"tentaco error {code}: {msg}")
+        self.code = code
+
+
+class TacoTensor(ctypes.Structure):
+    """ctypes mirror of ``taco_tensor_t`` (include/taco_tensor_t.h, 64 bytes)."""
+
+    _fields_ = [
+        ("order", c_int32),
+        ("dimensions", POINTER(c_int32)),
+        ("csize", c_int32),
+        ("mode_ordering", POINTER(c_int32)),
+        ("mode_types", POINTER(c_int)),
+        ("indices", POINTER(POINTER(POINTER(c_int32)))),
+        ("vals", POINTER(c_float)),
+        ("vals_size", c_int32),
+    ]
+
+
+class SynthParams(ctypes.Structure):
+    """ctypes mirror of ``tentaco_synth_params``."""
+
+    _fields_ = [
+        ("order", c_int),
+        ("dims", c_int32 * 5),
+        ("nnz", c_int64),
+        ("slice_s", c_double),
+        ("fanout_s", c_double),
+        ("mid_mu", c_double),
+        ("mid_sigma", c_double),
+        ("leaf_mu", c_double),
+        ("leaf_sigma", c_double),
+        ("leaf_cap", c_int32),
+        ("seed", c_uint64),
+    ]
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libtentaco.so once; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise TentacoError(ENOGPU, f"{LIB_PATH} not built (run __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    P = POINTER(TacoTensor)
+    sigs = {
+        "tentaco_last_error": (ctypes.c_char_p, []),
+        "tentaco_version": (c_int, []),
+        "tentaco_device_check": (c_int, []),
+        "tentaco_mttkrp3": (c_int, [P, P, P, P]),
+        "tentaco_mttkrp4": (c_int, [P, P, P, P, P]),
+        "tentaco_mttkrp5": (c_int, [P, P, P, P, P, P]),
+        "tentaco_mttkrp": (c_int, [c_int, P, P, POINTER(P), c_int]),
+        "tentaco_ttv": (c_int, [P, P, P]),
+        "tentaco_ttv_n": (c_int, [c_int, P, P, P]),
+        "get_gpu_timer_result": (c_float, []),
+        "mttkrp_gpu_taco": (None, [P, P, P, P]),
+        "ttv_csf_gpu_taco": (None, [P, P, P]),
+        "tentaco_num_variants": (c_int, []),
+        "tentaco_variant_name": (ctypes.c_char_p, [c_int]),
+        "tentaco_variant_params": (c_int, [c_int, POINTER(c_int), POINTER(c_int), POINTER(c_int)]),
+        "tentaco_default_variant": (c_int, []),
+        "tentaco_csf_upload": (c_int, [P, c_int, POINTER(c_void_p)]),
+        "tentaco_csf_free": (c_int, [c_void_p]),
+        "tentaco_csf_info": (c_int, [c_void_p, POINTER(c_int), POINTER(c_int64), POINTER(c_int32)]),
+        "tentaco_csf_reserve": (c_int, [c_void_p, c_int, c_int]),
+        "tentaco_mttkrp_dev": (c_int, [c_void_p, c_int, POINTER(c_void_p), c_void_p, c_int32, c_int, c_void_p]),
+        "tentaco_ttv_dev": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+        "tentaco_csf_autotune": (c_int, [c_void_p, c_int, POINTER(c_void_p), c_void_p, c_int32, c_int,
+                                         c_void_p, POINTER(c_int), POINTER(c_float)]),
+        "tentaco_csf_set_tuned": (c_int, [c_void_p, c_int, c_int]),
+        "tentaco_dev_malloc": (c_int, [POINTER(c_void_p), c_size_t]),
+        "tentaco_dev_free": (c_int, [c_void_p]),
+        "tentaco_memcpy_h2d": (c_int, [c_void_p, c_void_p, c_size_t]),
+        "tentaco_memcpy_d2h": (c_int, [c_void_p, c_void_p, c_size_t]),
+        "tentaco_dev_synchronize": (c_int, []),
+        "tentaco_timer_start": (c_int, [c_void_p]),
+        "tentaco_flush_cache": (c_int, [c_void_p]),
+        "tentaco_timer_stop": (c_int, [c_void_p, POINTER(c_float)]),
+        "tentaco_coo_to_csf": (c_int, [c_int, POINTER(c_int32), c_int64, POINTER(c_int32), POINTER(c_float), POINTER(c_void_p)]),
+        "tentaco_synth_csf": (c_int, [POINTER(SynthParams), POINTER(c_void_p)]),
+        "tentaco_host_csf_view": (P, [c_void_p]),
+        "tentaco_host_csf_nodes": (c_int64, [c_void_p, c_int]),
+        "tentaco_host_csf_free": (c_int, [c_void_p]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc: int) -> None:
+    if rc != OK:
+        msg = lib().tentaco_last_error()
+        raise TentacoError(rc, msg.decode() if msg else "")
+
+
+def device_check() -> None:
+    """Raise unless a gfx950 device is usable."""
+    _check(lib().tentaco_device_check())
+
+
+# ---------------------------------------------------------------------------
+# host CSF
+@dataclass
+class Csf:
+    """Host CSF tensor: level 0 is the root (pos[0] = [0, n0], crd[0]),
+    levels 1..order-1 hold pos/crd, vals the nonzero values (int32/FP32)."""
+
+    order: int
+    dims: tuple
+    pos: List[np.ndarray]
+    crd: List[np.ndarray]
+    vals: np.ndarray
+    _owner: object = field(default=None, repr=False, compare=False)
+
+    @property
+    def nnz(self) -> int:
+        return int(self.vals.shape[0])
+
+    def nodes(self, m: int) -> int:
+        return int(self.crd[m].shape[0])
+
+    def as_taco(self):
+        """taco_tensor_t view of this CSF (returns the struct and a keep-alive)."""
+        dims = np.ascontiguousarray(self.dims, dtype=np.int32)
+        keep = [dims]
+        lvl = (POINTER(c_int32) * 2 * self.order)()
+        ptrs = (POINTER(POINTER(c_int32)) * self.order)()
+        for m in range(self.order):
+            p = np.ascontiguousarray(self.pos[m], dtype=np.int32)
+            c = np.ascontiguousarray(self.crd[m], dtype=np.int32)
+            keep += [p, c]
+            lvl[m][0] = p.ctypes.data_as(POINTER(c_int32))
+            lvl[m][1] = c.ctypes.data_as(POINTER(c_int32))
+            ptrs[m] = ctypes.cast(lvl[m], POINTER(POINTER(c_int32)))
+        vals = np.ascontiguousarray(self.vals, dtype=np.float32)
+        keep += [vals, lvl, ptrs]
+        t = TacoTensor()
+        t.order = self.order
+        t.dimensions = dims.ctypes.data_as(POINTER(c_int32))
+        t.csize = 4
+        t.indices = ctypes.cast(ptrs, POINTER(POINTER(POINTER(c_int32))))
+        t.vals = vals.ctypes.data_as(POINTER(c_float))
+        t.vals_size = vals.shape[0]
+        return t, keep
+
+    @classmethod
+    def from_coo(cls, dims: Sequence[int], coords: np.ndarray, vals: np.ndarray) -> "Csf":
+        """Lexicographic COO -> CSF with duplicates summed (tentaco_coo_to_csf)."""
+        dims = np.ascontiguousarray(dims, dtype=np.int32)
+        coords = np.ascontiguousarray(coords, dtype=np.int32)
+        vals = np.ascontiguousarray(vals, dtype=np.float32)
+        order = dims.shape[0]
+        nnz = vals.shape[0]
+        assert coords.shape == (order, nnz)
+        h = c_void_p()
+        _check(
+            lib().tentaco_coo_to_csf(
+                order,
+                dims.ctypes.data_as(POINTER(c_int32)),
+                nnz,
+                coords.ctypes.data_as(POINTER(c_int32)),
+                vals.ctypes.data_as(POINTER(c_float)),
+                ctypes.byref(h),
+            )
+        )
+        return _wrap_host_csf(h)
+
+    @classmethod
+    def synth(cls, order: int, dims: Sequence[int], nnz: int, slice_s: float = 1.1,
+              fanout_s: float = 1.3, mid_mu: float = 1.0, mid_sigma: float = 1.0,
+              leaf_mu: float = 1.5, leaf_sigma: float = 1.0, leaf_cap: int = 10000,
+              seed: int = 0) -> "Csf":
+        """Synthetic Zipf / log-normal CSF (tentaco_synth_csf)."""
+        p = SynthParams()
+        p.order = order
+        for m in range(5):
+            p.dims[m] = int(dims[m]) if m < order else 0
+        p.nnz = int(nnz)
+        p.slice_s, p.fanout_s = slice_s, fanout_s
+        p.mid_mu, p.mid_sigma = mid_mu, mid_sigma
+        p.leaf_mu, p.leaf_sigma = leaf_mu, leaf_sigma
+        p.leaf_cap = leaf_cap
+        p.seed = seed
+        h = c_void_p()
+        _check(lib().tentaco_synth_csf(ctypes.byref(p), ctypes.byref(h)))
+        return _wrap_host_csf(h)
+
+
+class _HostCsfOwner:
+    def __init__(self, h):
+        self.h = h
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().tentaco_host_csf_free(self.h)
+        except Exception:
+            pass
+
+
+def _wrap_host_csf(h: c_void_p) -> Csf:
+    L = lib()
+    owner = _HostCsfOwner(h)
+    t = L.tentaco_host_csf_view(h).contents
+    order = t.order
+    dims = tuple(int(t.dimensions[m]) for m in range(order))
+    nodes = [int(L.tentaco_host_csf_nodes(h, m)) for m in range(order)]
+    pos, crd = [], []
+    parents = 1
+    for m in range(order):
+        np_len = 2 if m == 0 else parents + 1
+        pos.append(np.ctypeslib.as_array(t.indices[m][0], shape=(np_len,)))
+        crd.append(np.ctypeslib.as_array(t.indices[m][1], shape=(max(nodes[m], 0),)) if nodes[m] > 0
+                   else np.zeros(0, np.int32))
+        parents = nodes[m]
+    nnz = nodes[order - 1]
+    vals = np.ctypeslib.as_array(t.vals, shape=(nnz,)) if nnz > 0 else np.zeros(0, np.float32)
+    return Csf(order, dims, pos, crd, vals, _owner=owner)
+
+
+def dense_taco(a: np.ndarray):
+    """taco_tensor_t view of a dense row-major FP32 matrix or vector."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    dims = np.ascontiguousarray(a.shape, dtype=np.int32)
+    t = TacoTensor()
+    t.order = a.ndim
+    t.dimensions = dims.ctypes.data_as(POINTER(c_int32))
+    t.csize = 4
+    t.vals = a.ctypes.data_as(POINTER(c_float))
+    t.vals_size = a.size
+    return t, [a, dims]
+
+
+# ---------------------------------------------------------------------------
+# host-pointer entry points (layer 1)
+def mttkrp(B: Csf, factors: Sequence[np.ndarray], variant: int = VARIANT_AUTO,
+           out_rows: Optional[int] = None) -> np.ndarray:
+    """A = MTTKRP(B, factors) on the GPU; factors[m-1] is the factor of level m
+    (rows x R, FP32). Mirrors tentaco_mttkrp (gpu_kernels.cuh:5)."""
+    assert len(factors) == B.order - 1
+    R = int(factors[0].shape[1])
+    rows = B.dims[0] if out_rows is None else out_rows
+    A = np.empty((rows, R), dtype=np.float32)
+    tA, kA = dense_taco(A)
+    tB, kB = B.as_taco()
+    fs = [dense_taco(f) for f in factors]
+    arr = (POINTER(TacoTensor) * len(fs))(*[ctypes.pointer(t) for t, _ in fs])
+    _check(lib().tentaco_mttkrp(B.order, ctypes.byref(tA), ctypes.byref(tB), arr, variant))
+    del kA, kB
+    return A
+
+
+def ttv(B: Csf, c: np.ndarray) -> np.ndarray:
+    """y = TTV(B, c) over the last mode (ttv_csf_gpu_taco, gpu_kernels.cuh:6)."""
+    nf = B.nodes(B.order - 2)
+    y = np.empty(nf, dtype=np.float32)
+    tA, kA = dense_taco(y)
+    tB, kB = B.as_taco()
+    tc, kc = dense_taco(np.ascontiguousarray(c, dtype=np.float32))
+    _check(lib().tentaco_ttv_n(B.order, ctypes.byref(tA), ctypes.byref(tB), ctypes.byref(tc)))
+    del kA, kB, kc
+    return y
+
+
+def last_timer_ms() -> float:
+    """get_gpu_timer_result(): hot-region ms of the last host-pointer call."""
+    return float(lib().get_gpu_timer_result())
+
+
+def variants() -> List[str]:
+    L = lib()
+    return [L.tentaco_variant_name(v).decode() for v in range(L.tentaco_num_variants())]
+
+
+def variant_params(v: int):
+    fam, blk, grp = c_int(), c_int(), c_int()
+    _check(lib().tentaco_variant_params(v, ctypes.byref(fam), ctypes.byref(blk), ctypes.byref(grp)))
+    return fam.value, blk.value, grp.value
+
+
+def default_variant() -> int:
+    return int(lib().tentaco_default_variant())
+
+
+# ---------------------------------------------------------------------------
+# device-resident API (layer 3)
+class DeviceArray:
+    """A device buffer owned by the library's HIP runtime."""
+
+    def __init__(self, nbytes: int, dtype=np.float32, shape=None):
+        self.ptr = c_void_p()
+        self.nbytes = int(nbytes)
+        self.dtype = np.dtype(dtype)
+        self.shape = shape if shape is not None else (self.nbytes // self.dtype.itemsize,)
+        _check(lib().tentaco_dev_malloc(ctypes.byref(self.ptr), max(self.nbytes, 4)))
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.nbytes, a.dtype, a.shape)
+        if a.nbytes:
+            _check(lib().tentaco_memcpy_h2d(d.ptr, a.ctypes.data_as(c_void_p), a.nbytes))
+        return d
+
+    @classmethod
+    def empty(cls, shape, dtype=np.float32) -> "DeviceArray":
+        shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        n = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        return cls(n, dtype, shape)
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype)
+        if self.nbytes:
+            _check(lib().tentaco_memcpy_d2h(out.ctypes.data_as(c_void_p), self.ptr, self.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().tentaco_dev_free(self.ptr)
+            self.ptr = c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class DeviceCsf:
+    """Device-resident CSF handle (tentaco_csf_upload)."""
+
+    def __init__(self, B: Csf):
+        t, keep = B.as_taco()
+        self.h = c_void_p()
+        _check(lib().tentaco_csf_upload(ctypes.byref(t), B.order, ctypes.byref(self.h)))
+        del keep
+        self.order = B.order
+        self.dims = B.dims
+        self.nnz = B.nnz
+        self.nodes = [B.nodes(m) for m in range(B.order)]
+
+    def reserve(self, R: int, variant: int = VARIANT_AUTO):
+        _check(lib().tentaco_csf_reserve(self.h, R, variant))
+
+    def mttkrp(self, factors: Sequence[DeviceArray], A: DeviceArray, R: int,
+               variant: int = VARIANT_AUTO, stream=None, a_rows: Optional[int] = None):
+        arr = (c_void_p * len(factors))(*[f.ptr for f in factors])
+        rows = self.dims[0] if a_rows is None else a_rows
+        _check(lib().tentaco_mttkrp_dev(self.h, R, arr, A.ptr, rows, variant, stream))
+
+    def autotune(self, factors: Sequence[DeviceArray], A: DeviceArray, R: int, reps: int = 5,
+                 stream=None, a_rows: Optional[int] = None):
+        """Time every variant; the fastest becomes this handle's AUTO choice.
+        Returns (best_variant, per-variant median ms, -1 where unsupported)."""
+        arr = (c_void_p * len(factors))(*[f.ptr for f in factors])
+        rows = self.dims[0] if a_rows is None else a_rows
+        n = lib().tentaco_num_variants()
+        times = (c_float * n)()
+        best = c_int()
+        _check(lib().tentaco_csf_autotune(self.h, R, arr, A.ptr, rows, reps, stream,
+                                          ctypes.byref(best), times))
+        return best.value, [float(t) for t in times]
+
+    def set_tuned(self, R: int, variant: int):
+        _check(lib().tentaco_csf_set_tuned(self.h, R, variant))
+
+    def ttv(self, c: DeviceArray, y: DeviceArray, stream=None):
+        _check(lib().tentaco_ttv_dev(self.h, c.ptr, y.ptr, stream))
+
+    def close(self):
+        if self.h:
+            lib().tentaco_csf_free(self.h)
+            self.h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def synchronize():
+    _check(lib().tentaco_dev_synchronize())
+
+
+def flush_cache(stream=None):
+    """Overwrite 512 MiB on the device so the next kernel starts cold."""
+    _check(lib().tentaco_flush_cache(stream))
+
+
+def timer_start(stream=None):
+    _check(lib().tentaco_timer_start(stream))
+
+
+def timer_stop(stream=None) -> float:
+    ms = c_float()
+    _check(lib().tentaco_timer_stop(stream, ctypes.byref(ms)))
+    return float(ms.value)

@@ -1,0 +1,730 @@
+// C ABI of the tentaco library: validation, device-resident CSF handles,
+// per-call staging for the host-pointer entry points, hipEvent timing and the
+// reference's symbol names. Kernels live in mttkrp.hip / ttv.hip.
+
+#include <algorithm>
+#include <atomic>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+#include "tuning.h"
+
+// ---------------------------------------------------------------------------
+// errors
+namespace tt {
+namespace {
+thread_local std::string g_err;
+}
+
+int fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+void clear_error() { g_err.clear(); }
+
+}  // namespace tt
+
+using tt::fail;
+
+// ---------------------------------------------------------------------------
+// device-resident CSF
+struct tentaco_csf {
+  int order = 0;
+  int32_t dims[tt::kMaxOrder] = {};
+  int32_t nodes[tt::kMaxOrder] = {};
+  int32_t max_crd[tt::kMaxOrder] = {};  // largest coordinate per level, -1 if none
+  bool dense_root = false;
+  int32_t* d_crd0 = nullptr;
+  int32_t* d_pos[tt::kMaxOrder] = {};
+  int32_t* d_crd[tt::kMaxOrder] = {};
+  float* d_vals = nullptr;
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+  tt::TuneTable tune;  // per-rank best variant found by the autotuner
+  tt::CsfStats stats;
+
+  tt::CsfDev view() const {
+    tt::CsfDev v;
+    v.order = order;
+    v.nnz = nodes[order - 1];
+    for (int m = 0; m < order; ++m) v.nodes[m] = nodes[m];
+    v.crd0 = d_crd0;
+    for (int m = 1; m < order; ++m) {
+      v.pos[m] = d_pos[m];
+      v.crd[m] = d_crd[m];
+    }
+    v.vals = d_vals;
+    return v;
+  }
+};
+
+namespace tt {
+namespace {
+
+constexpr int64_t kMaxNnz = (int64_t(1) << 31) - (int64_t(1) << 16);
+
+// Host-side structural validation; fills node counts, max coordinates and
+// the dense-root flag.
+int validate_csf(const taco_tensor_t* B, int order, tentaco_csf* h) {
+  if (!B) return fail(TENTACO_EINVAL, "B is NULL");
+  if (order < 2 || order > kMaxOrder)
+    return fail(TENTACO_EUNSUPPORTED, "order %d outside 2..%d", order, kMaxOrder);
+  if (!B->dimensions || !B->indices || !B->vals)
+    return fail(TENTACO_EINVAL, "B has NULL dimensions/indices/vals");
+  for (int m = 0; m < order; ++m) {
+    if (!B->indices[m] || !B->indices[m][0] || !B->indices[m][1])
+      return fail(TENTACO_EINVAL, "B->indices[%d] incomplete", m);
+    if (B->dimensions[m] <= 0)
+      return fail(TENTACO_EINVAL, "B->dimensions[%d] = %d", m, B->dimensions[m]);
+    h->dims[m] = B->dimensions[m];
+  }
+  const int32_t* pos0 = B->indices[0][0];
+  if (pos0[0] != 0 || pos0[1] < 0)
+    return fail(TENTACO_EINVAL, "root pos must be {0, n0}, got {%d, %d}", pos0[0], pos0[1]);
+  const int32_t n0 = pos0[1];
+  if (n0 > h->dims[0])
+    return fail(TENTACO_EINVAL, "root has %d nodes > dimension %d", n0, h->dims[0]);
+  h->nodes[0] = n0;
+  const int32_t* crd0 = B->indices[0][1];
+  bool dense = (n0 == h->dims[0]);
+  int32_t prev = -1;
+  for (int32_t x = 0; x < n0; ++x) {
+    const int32_t v = crd0[x];
+    if (v <= prev || v >= h->dims[0])
+      return fail(TENTACO_EINVAL, "root crd[%d]=%d not strictly increasing in [0,%d)", x, v,
+                  h->dims[0]);
+    if (v != x) dense = false;
+    prev = v;
+  }
+  h->max_crd[0] = prev;
+  h->dense_root = dense;
+  int64_t parents = n0;
+  for (int m = 1; m < order; ++m) {
+    const int32_t* pos = B->indices[m][0];
+    const int32_t* crd = B->indices[m][1];
+    if (pos[0] != 0) return fail(TENTACO_EINVAL, "level %d pos[0] = %d", m, pos[0]);
+    for (int64_t x = 0; x < parents; ++x)
+      if (pos[x + 1] < pos[x])
+        return fail(TENTACO_EINVAL, "level %d pos decreases at %lld", m, (long long)x);
+    const int64_t nodes = pos[parents];
+    if (nodes > kMaxNnz)
+      return fail(TENTACO_EUNSUPPORTED, "level %d has %lld nodes (int32 index limit)", m,
+                  (long long)nodes);
+    int32_t mx = -1;
+    const int32_t dim = h->dims[m];
+    for (int64_t x = 0; x < nodes; ++x) {
+      const int32_t v = crd[x];
+      if (v < 0 || v >= dim)
+        return fail(TENTACO_EINVAL, "level %d crd[%lld]=%d outside [0,%d)", m, (long long)x, v,
+                    dim);
+      mx = v > mx ? v : mx;
+    }
+    h->max_crd[m] = mx;
+    h->nodes[m] = int32_t(nodes);
+    parents = nodes;
+  }
+  // per-slice nonzero counts (descend the pos arrays from each root node)
+  CsfStats& st = h->stats;
+  st.nnz = h->nodes[order - 1];
+  st.n0 = n0;
+  auto start_of = [&](int64_t x) {
+    for (int m = 1; m < order; ++m) x = B->indices[m][0][x];
+    return x;
+  };
+  int64_t prev_start = start_of(0);
+  for (int64_t x = 0; x < n0; ++x) {
+    const int64_t next = start_of(x + 1);
+    const int64_t cnt = next - prev_start;
+    if (cnt > 0) ++st.nonempty_slices;
+    if (cnt > st.max_slice_nnz) st.max_slice_nnz = cnt;
+    prev_start = next;
+  }
+  return TENTACO_OK;
+}
+
+template <class T>
+int upload_array(T** dst, const T* src, int64_t n) {
+  *dst = nullptr;
+  const size_t bytes = size_t(std::max<int64_t>(n, 1)) * sizeof(T);
+  if (hipMalloc(reinterpret_cast<void**>(dst), bytes) != hipSuccess) {
+    *dst = nullptr;
+    return fail(TENTACO_ENOMEM, "hipMalloc of %zu bytes failed", bytes);
+  }
+  if (n > 0) TT_HIP_CHECK(hipMemcpy(*dst, src, size_t(n) * sizeof(T), hipMemcpyHostToDevice));
+  return TENTACO_OK;
+}
+
+void free_handle(tentaco_csf* h) {
+  if (!h) return;
+  (void)hipFree(h->d_crd0);
+  for (int m = 0; m < kMaxOrder; ++m) {
+    (void)hipFree(h->d_pos[m]);
+    (void)hipFree(h->d_crd[m]);
+  }
+  (void)hipFree(h->d_vals);
+  (void)hipFree(h->ws);
+  delete h;
+}
+
+int reserve_ws(tentaco_csf* h, size_t bytes) {
+  if (bytes <= h->ws_bytes) return TENTACO_OK;
+  (void)hipFree(h->ws);
+  h->ws = nullptr;
+  h->ws_bytes = 0;
+  if (hipMalloc(&h->ws, bytes) != hipSuccess) {
+    h->ws = nullptr;
+    return fail(TENTACO_ENOMEM, "workspace hipMalloc of %zu bytes failed", bytes);
+  }
+  h->ws_bytes = bytes;
+  return TENTACO_OK;
+}
+
+int resolve_variant(const tentaco_csf* h, int R, int variant) {
+  if (variant == TENTACO_VARIANT_DEFAULT) return default_variant();
+  if (variant == TENTACO_VARIANT_AUTO) {
+    const int tuned = h->tune.lookup(R);
+    if (tuned >= 0) return tuned;
+    return auto_variant(h->stats, R);
+  }
+  return variant;
+}
+
+// ---- hot-region timer (GPUTimer, gpu_library.h:949-988) -------------------
+std::atomic<float> g_last_ms{-1.f};
+std::mutex g_timer_mu;
+hipEvent_t g_ev0 = nullptr, g_ev1 = nullptr;
+
+int ensure_events() {
+  if (!g_ev0) TT_HIP_CHECK(hipEventCreate(&g_ev0));
+  if (!g_ev1) TT_HIP_CHECK(hipEventCreate(&g_ev1));
+  return TENTACO_OK;
+}
+
+// ---- per-call staging for the host-pointer entry points ------------------
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() { (void)hipFree(p); }
+};
+
+int check_factor(const taco_tensor_t* F, int level, const tentaco_csf* h, int R) {
+  if (!F || !F->dimensions || !F->vals)
+    return fail(TENTACO_EINVAL, "factor of level %d is NULL/incomplete", level);
+  if (F->dimensions[1] != R)
+    return fail(TENTACO_EINVAL, "factor of level %d has %d columns, expected R=%d", level,
+                F->dimensions[1], R);
+  if (F->dimensions[0] <= h->max_crd[level])
+    return fail(TENTACO_EINVAL, "factor of level %d has %d rows but coordinates reach %d",
+                level, F->dimensions[0], h->max_crd[level]);
+  return TENTACO_OK;
+}
+
+int host_mttkrp(int order, taco_tensor_t* A, taco_tensor_t* B,
+                taco_tensor_t* const* factors, int variant) {
+  clear_error();
+  if (!A || !A->dimensions || !A->vals) return fail(TENTACO_EINVAL, "A is NULL/incomplete");
+  if (!factors) return fail(TENTACO_EINVAL, "factors is NULL");
+  tentaco_csf* h = nullptr;
+  int rc = tentaco_csf_upload(B, order, &h);
+  if (rc) return rc;
+  struct Guard {
+    tentaco_csf* h;
+    ~Guard() { free_handle(h); }
+  } guard{h};
+  const int R = A->dimensions[1];
+  const int32_t a_rows = A->dimensions[0];
+  if (R <= 0) return fail(TENTACO_EINVAL, "A has %d columns", R);
+  if (a_rows <= h->max_crd[0])
+    return fail(TENTACO_EINVAL, "A has %d rows but root coordinates reach %d", a_rows,
+                h->max_crd[0]);
+  DevBuf fb[kMaxOrder];
+  const float* fdev[kMaxOrder] = {};
+  for (int m = 1; m < order; ++m) {
+    rc = check_factor(factors[m - 1], m, h, R);
+    if (rc) return rc;
+    float* d = nullptr;
+    rc = upload_array(&d, factors[m - 1]->vals, int64_t(factors[m - 1]->dimensions[0]) * R);
+    fb[m].p = d;
+    if (rc) return rc;
+    fdev[m - 1] = d;
+  }
+  DevBuf abuf;
+  const size_t abytes = size_t(a_rows) * size_t(R) * sizeof(float);
+  if (hipMalloc(&abuf.p, std::max<size_t>(abytes, 4)) != hipSuccess)
+    return fail(TENTACO_ENOMEM, "hipMalloc of A (%zu bytes) failed", abytes);
+  const int v = resolve_variant(h, R, variant);
+  const Variant vr = variant_of(v);
+  if (vr.family < 0) return fail(TENTACO_EINVAL, "variant %d out of range", variant);
+  rc = reserve_ws(h, mttkrp_workspace_bytes(h->view(), R, vr));
+  if (rc) return rc;
+  TT_HIP_CHECK(hipDeviceSynchronize());
+  {
+    std::lock_guard<std::mutex> lk(g_timer_mu);
+    rc = ensure_events();
+    if (rc) return rc;
+    TT_HIP_CHECK(hipEventRecord(g_ev0, nullptr));
+    rc = tentaco_mttkrp_dev(h, R, fdev, static_cast<float*>(abuf.p), a_rows, v, nullptr);
+    if (rc) return rc;
+    TT_HIP_CHECK(hipEventRecord(g_ev1, nullptr));
+    TT_HIP_CHECK(hipEventSynchronize(g_ev1));
+    float ms = -1.f;
+    TT_HIP_CHECK(hipEventElapsedTime(&ms, g_ev0, g_ev1));
+    g_last_ms.store(ms);
+  }
+  TT_HIP_CHECK(hipMemcpy(A->vals, abuf.p, abytes, hipMemcpyDeviceToHost));
+  return TENTACO_OK;
+}
+
+int host_ttv(int order, taco_tensor_t* A, taco_tensor_t* B, taco_tensor_t* c) {
+  clear_error();
+  if (!A || !A->vals) return fail(TENTACO_EINVAL, "A is NULL/incomplete");
+  if (!c || !c->dimensions || !c->vals) return fail(TENTACO_EINVAL, "c is NULL/incomplete");
+  tentaco_csf* h = nullptr;
+  int rc = tentaco_csf_upload(B, order, &h);
+  if (rc) return rc;
+  struct Guard {
+    tentaco_csf* h;
+    ~Guard() { free_handle(h); }
+  } guard{h};
+  if (c->dimensions[0] <= h->max_crd[order - 1])
+    return fail(TENTACO_EINVAL, "c has %d entries but coordinates reach %d", c->dimensions[0],
+                h->max_crd[order - 1]);
+  const int32_t nf = h->nodes[order - 2];
+  if (A->dimensions && A->dimensions[0] < nf)
+    return fail(TENTACO_EINVAL, "A has %d entries, B has %d fibers", A->dimensions[0], nf);
+  DevBuf cbuf, ybuf;
+  float* cd = nullptr;
+  rc = upload_array(&cd, c->vals, c->dimensions[0]);
+  cbuf.p = cd;
+  if (rc) return rc;
+  if (hipMalloc(&ybuf.p, std::max<size_t>(size_t(nf) * 4, 4)) != hipSuccess)
+    return fail(TENTACO_ENOMEM, "hipMalloc of y failed");
+  TT_HIP_CHECK(hipDeviceSynchronize());
+  {
+    std::lock_guard<std::mutex> lk(g_timer_mu);
+    rc = ensure_events();
+    if (rc) return rc;
+    TT_HIP_CHECK(hipEventRecord(g_ev0, nullptr));
+    rc = tentaco_ttv_dev(h, cd, static_cast<float*>(ybuf.p), nullptr);
+    if (rc) return rc;
+    TT_HIP_CHECK(hipEventRecord(g_ev1, nullptr));
+    TT_HIP_CHECK(hipEventSynchronize(g_ev1));
+    float ms = -1.f;
+    TT_HIP_CHECK(hipEventElapsedTime(&ms, g_ev0, g_ev1));
+    g_last_ms.store(ms);
+  }
+  if (nf > 0) TT_HIP_CHECK(hipMemcpy(A->vals, ybuf.p, size_t(nf) * 4, hipMemcpyDeviceToHost));
+  return TENTACO_OK;
+}
+
+// Legacy void entry points report failures on stderr (as gpuErrchk does,
+// gpu_library.h:22-30) and through tentaco_last_error(), without exiting.
+void legacy_report(const char* fn, int rc) {
+  if (rc) std::fprintf(stderr, "tentaco %s failed (%d): %s\n", fn, rc, g_err.c_str());
+}
+
+int variant_index(int family, int block, int group) {
+  int bi = -1, gi = -1;
+  for (int i = 0; i < kNumGridBlocks; ++i)
+    if (kGridBlocks[i] == block) bi = i;
+  for (int i = 0; i < kNumGridGroups; ++i)
+    if (kGridGroups[i] == group) gi = i;
+  if (bi < 0 || gi < 0) return default_variant();
+  return family * kNumGridBlocks * kNumGridGroups + bi * kNumGridGroups + gi;
+}
+
+}  // namespace
+}  // namespace tt
+
+using namespace tt;
+
+extern "C" {
+
+const char* tentaco_last_error(void) { return tt::g_err.c_str(); }
+
+int tentaco_version(void) { return 10000; }
+
+int tentaco_device_check(void) {
+  clear_error();
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+    return fail(TENTACO_ENOGPU, "no HIP device visible");
+  int dev = 0;
+  TT_HIP_CHECK(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  TT_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(TENTACO_ENOGPU, "device %d is %s, this build targets gfx950", dev,
+                prop.gcnArchName);
+  return TENTACO_OK;
+}
+
+int tentaco_csf_upload(const taco_tensor_t* B, int order, tentaco_csf** out) {
+  clear_error();
+  if (!out) return fail(TENTACO_EINVAL, "out is NULL");
+  *out = nullptr;
+  tentaco_csf* h = new tentaco_csf;
+  h->order = order;
+  int rc = validate_csf(B, order, h);
+  if (rc) {
+    delete h;
+    return rc;
+  }
+  rc = upload_array(&h->d_crd0, B->indices[0][1], h->nodes[0]);
+  for (int m = 1; m < order && !rc; ++m) {
+    rc = upload_array(&h->d_pos[m], B->indices[m][0], int64_t(h->nodes[m - 1]) + 1);
+    if (!rc) rc = upload_array(&h->d_crd[m], B->indices[m][1], h->nodes[m]);
+  }
+  if (!rc) rc = upload_array(&h->d_vals, B->vals, h->nodes[order - 1]);
+  if (rc) {
+    free_handle(h);
+    return rc;
+  }
+  *out = h;
+  return TENTACO_OK;
+}
+
+int tentaco_csf_free(tentaco_csf* h) {
+  free_handle(h);
+  return TENTACO_OK;
+}
+
+int tentaco_csf_info(const tentaco_csf* h, int* order, int64_t* nnz, int32_t* nodes) {
+  if (!h) return fail(TENTACO_EINVAL, "handle is NULL");
+  if (order) *order = h->order;
+  if (nnz) *nnz = h->nodes[h->order - 1];
+  if (nodes)
+    for (int m = 0; m < kMaxOrder; ++m) nodes[m] = m < h->order ? h->nodes[m] : 0;
+  return TENTACO_OK;
+}
+
+int tentaco_csf_reserve(tentaco_csf* h, int R, int variant) {
+  clear_error();
+  if (!h) return fail(TENTACO_EINVAL, "handle is NULL");
+  size_t need = ttv_workspace_bytes(h->view());
+  if (variant == TENTACO_VARIANT_AUTO && h->tune.lookup(R) < 0) {
+    // AUTO may pick any variant later: reserve for the largest need
+    for (int v = 0; v < kNumVariants; ++v)
+      need = std::max(need, mttkrp_workspace_bytes(h->view(), R, variant_of(v)));
+  } else {
+    const Variant vr = variant_of(resolve_variant(h, R, variant));
+    if (vr.family < 0) return fail(TENTACO_EINVAL, "variant %d out of range", variant);
+    need = std::max(need, mttkrp_workspace_bytes(h->view(), R, vr));
+  }
+  return reserve_ws(h, need);
+}
+
+int tentaco_mttkrp_dev(tentaco_csf* h, int R, const float* const* factors_dev, float* A_dev,
+                       int32_t a_rows, int variant, void* stream) {
+  if (!h) return fail(TENTACO_EINVAL, "handle is NULL");
+  if (!A_dev || !factors_dev) return fail(TENTACO_EINVAL, "NULL device buffer");
+  if (R <= 0) return fail(TENTACO_EINVAL, "R=%d", R);
+  if (a_rows <= h->max_crd[0])
+    return fail(TENTACO_EINVAL, "A has %d rows but root coordinates reach %d", a_rows,
+                h->max_crd[0]);
+  FactorsDev F;
+  for (int m = 1; m < h->order; ++m) {
+    if (!factors_dev[m - 1]) return fail(TENTACO_EINVAL, "factor of level %d is NULL", m);
+    F.M[m] = factors_dev[m - 1];
+  }
+  const int v = resolve_variant(h, R, variant);
+  const Variant vr = variant_of(v);
+  if (vr.family < 0) return fail(TENTACO_EINVAL, "variant %d out of range", variant);
+  const CsfDev B = h->view();
+  int rc = reserve_ws(h, mttkrp_workspace_bytes(B, R, vr));
+  if (rc) return rc;
+  const bool dense = h->dense_root && a_rows == h->nodes[0];
+  return launch_mttkrp(B, F, R, A_dev, a_rows, dense, vr, h->ws,
+                       static_cast<hipStream_t>(stream));
+}
+
+int tentaco_csf_autotune(tentaco_csf* h, int R, const float* const* factors_dev, float* A_dev,
+                         int32_t a_rows, int reps, void* stream, int* best, float* times_ms) {
+  clear_error();
+  if (!h) return fail(TENTACO_EINVAL, "handle is NULL");
+  if (reps < 1) reps = 1;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  hipEvent_t e0, e1;
+  TT_HIP_CHECK(hipEventCreate(&e0));
+  TT_HIP_CHECK(hipEventCreate(&e1));
+  struct Ev {
+    hipEvent_t a, b;
+    ~Ev() {
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
+    }
+  } ev{e0, e1};
+  int best_v = -1;
+  float best_ms = 0.f;
+  std::vector<float> t(static_cast<size_t>(reps));
+  for (int v = 0; v < kNumVariants; ++v) {
+    float med = -1.f;
+    int rc = tentaco_mttkrp_dev(h, R, factors_dev, A_dev, a_rows, v, stream);  // warm-up + ws
+    if (rc == TENTACO_OK) {
+      for (int r = 0; r < reps && rc == TENTACO_OK; ++r) {
+        TT_HIP_CHECK(hipEventRecord(e0, st));
+        rc = tentaco_mttkrp_dev(h, R, factors_dev, A_dev, a_rows, v, stream);
+        TT_HIP_CHECK(hipEventRecord(e1, st));
+        TT_HIP_CHECK(hipEventSynchronize(e1));
+        TT_HIP_CHECK(hipEventElapsedTime(&t[size_t(r)], e0, e1));
+      }
+      if (rc == TENTACO_OK) {
+        std::sort(t.begin(), t.end());
+        med = t[t.size() / 2];
+      }
+    }
+    if (rc == TENTACO_EUNSUPPORTED) clear_error();
+    else if (rc != TENTACO_OK) return rc;
+    if (times_ms) times_ms[v] = med;
+    if (med >= 0.f && (best_v < 0 || med < best_ms)) {
+      best_v = v;
+      best_ms = med;
+    }
+  }
+  if (best_v < 0) return fail(TENTACO_EUNSUPPORTED, "no variant ran");
+  h->tune.set(R, best_v);
+  if (best) *best = best_v;
+  return TENTACO_OK;
+}
+
+int tentaco_csf_set_tuned(tentaco_csf* h, int R, int variant) {
+  if (!h) return fail(TENTACO_EINVAL, "handle is NULL");
+  if (variant_of(variant).family < 0) return fail(TENTACO_EINVAL, "variant %d out of range", variant);
+  h->tune.set(R, variant);
+  return TENTACO_OK;
+}
+
+int tentaco_ttv_dev(tentaco_csf* h, const float* c_dev, float* A_dev, void* stream) {
+  if (!h) return fail(TENTACO_EINVAL, "handle is NULL");
+  if (!c_dev || !A_dev) return fail(TENTACO_EINVAL, "NULL device buffer");
+  const CsfDev B = h->view();
+  int rc = reserve_ws(h, ttv_workspace_bytes(B));
+  if (rc) return rc;
+  return launch_ttv(B, c_dev, A_dev, h->ws, static_cast<hipStream_t>(stream));
+}
+
+int tentaco_mttkrp3(taco_tensor_t* A, taco_tensor_t* B, taco_tensor_t* C, taco_tensor_t* D) {
+  taco_tensor_t* f[2] = {C, D};
+  return host_mttkrp(3, A, B, f, TENTACO_VARIANT_AUTO);
+}
+
+int tentaco_mttkrp4(taco_tensor_t* A, taco_tensor_t* B, taco_tensor_t* C, taco_tensor_t* D,
+                    taco_tensor_t* E) {
+  taco_tensor_t* f[3] = {C, D, E};
+  return host_mttkrp(4, A, B, f, TENTACO_VARIANT_AUTO);
+}
+
+int tentaco_mttkrp5(taco_tensor_t* A, taco_tensor_t* B, taco_tensor_t* C, taco_tensor_t* D,
+                    taco_tensor_t* E, taco_tensor_t* F) {
+  taco_tensor_t* f[4] = {C, D, E, F};
+  return host_mttkrp(5, A, B, f, TENTACO_VARIANT_AUTO);
+}
+
+int tentaco_mttkrp(int order, taco_tensor_t* A, taco_tensor_t* B,
+                   taco_tensor_t* const* factors, int variant) {
+  return host_mttkrp(order, A, B, factors, variant);
+}
+
+int tentaco_ttv(taco_tensor_t* A, taco_tensor_t* B, taco_tensor_t* c) {
+  return host_ttv(3, A, B, c);
+}
+
+int tentaco_ttv_n(int order, taco_tensor_t* A, taco_tensor_t* B, taco_tensor_t* c) {
+  return host_ttv(order, A, B, c);
+}
+
+float get_gpu_timer_result(void) { return g_last_ms.load(); }
+
+void mttkrp_gpu_taco(taco_tensor_t* A, taco_tensor_t* B, taco_tensor_t* C, taco_tensor_t* D) {
+  taco_tensor_t* f[2] = {C, D};
+  legacy_report("mttkrp_gpu_taco", host_mttkrp(3, A, B, f, TENTACO_VARIANT_DEFAULT));
+}
+
+void ttv_csf_gpu_taco(taco_tensor_t* A, taco_tensor_t* B, taco_tensor_t* C) {
+  legacy_report("ttv_csf_gpu_taco", host_ttv(3, A, B, C));
+}
+
+#define TT_LEGACY(name, fam, tb, grp)                                                \
+  void name(taco_tensor_t* A, taco_tensor_t* B, taco_tensor_t* C, taco_tensor_t* D) { \
+    taco_tensor_t* f[2] = {C, D};                                                    \
+    legacy_report(#name, host_mttkrp(3, A, B, f, variant_index(fam, tb, grp)));      \
+  }
+// eb_sr_{TB}_{g}_{c}: edge-serial, block TB, run length g
+TT_LEGACY(mttkrp_32_eb_sr_512_4_1, 0, 512, 4)
+TT_LEGACY(mttkrp_32_eb_sr_256_4_1, 0, 256, 4)
+TT_LEGACY(mttkrp_32_eb_sr_256_4_2, 0, 256, 4)
+TT_LEGACY(mttkrp_32_eb_sr_256_4_4, 0, 256, 4)
+TT_LEGACY(mttkrp_32_eb_sr_256_4_8, 0, 256, 4)
+TT_LEGACY(mttkrp_32_eb_sr_256_8_1, 0, 256, 8)
+TT_LEGACY(mttkrp_32_eb_sr_256_8_2, 0, 256, 8)
+TT_LEGACY(mttkrp_32_eb_sr_256_8_4, 0, 256, 8)
+TT_LEGACY(mttkrp_32_eb_sr_256_8_8, 0, 256, 8)
+TT_LEGACY(mttkrp_32_eb_sr_256_16_1, 0, 256, 16)
+TT_LEGACY(mttkrp_32_eb_sr_256_16_2, 0, 256, 16)
+TT_LEGACY(mttkrp_32_eb_sr_256_16_4, 0, 256, 16)
+TT_LEGACY(mttkrp_32_eb_sr_256_16_8, 0, 256, 16)
+TT_LEGACY(mttkrp_32_eb_sr_256_32_1, 0, 256, 32)
+TT_LEGACY(mttkrp_32_eb_sr_256_32_2, 0, 256, 32)
+TT_LEGACY(mttkrp_32_eb_sr_256_32_4, 0, 256, 32)
+TT_LEGACY(mttkrp_32_eb_sr_256_32_8, 0, 256, 32)
+TT_LEGACY(mttkrp_32_eb_sr_512_4_2, 0, 512, 4)
+TT_LEGACY(mttkrp_32_eb_sr_512_4_4, 0, 512, 4)
+TT_LEGACY(mttkrp_32_eb_sr_512_4_8, 0, 512, 4)
+TT_LEGACY(mttkrp_32_eb_sr_512_8_1, 0, 512, 8)
+TT_LEGACY(mttkrp_32_eb_sr_512_8_2, 0, 512, 8)
+TT_LEGACY(mttkrp_32_eb_sr_512_8_4, 0, 512, 8)
+TT_LEGACY(mttkrp_32_eb_sr_512_8_8, 0, 512, 8)
+TT_LEGACY(mttkrp_32_eb_sr_512_16_1, 0, 512, 16)
+TT_LEGACY(mttkrp_32_eb_sr_512_16_2, 0, 512, 16)
+TT_LEGACY(mttkrp_32_eb_sr_512_16_4, 0, 512, 16)
+TT_LEGACY(mttkrp_32_eb_sr_512_16_8, 0, 512, 16)
+TT_LEGACY(mttkrp_32_eb_sr_512_32_1, 0, 512, 32)
+TT_LEGACY(mttkrp_32_eb_sr_512_32_2, 0, 512, 32)
+TT_LEGACY(mttkrp_32_eb_sr_512_32_4, 0, 512, 32)
+TT_LEGACY(mttkrp_32_eb_sr_512_32_8, 0, 512, 32)
+// eb_pr_{TB}_{Nnz}_{gs}: edge-parallel, block TB, reduction group gs
+TT_LEGACY(mttkrp_32_eb_pr_256_64_32_manual, 1, 256, 32)
+TT_LEGACY(mttkrp_64_eb_pr_256_32_32, 1, 256, 32)
+TT_LEGACY(mttkrp_32_eb_pr_256_32_4, 1, 256, 4)
+TT_LEGACY(mttkrp_32_eb_pr_256_32_8, 1, 256, 8)
+TT_LEGACY(mttkrp_32_eb_pr_256_32_16, 1, 256, 16)
+TT_LEGACY(mttkrp_32_eb_pr_256_32_32, 1, 256, 32)
+TT_LEGACY(mttkrp_32_eb_pr_256_64_4, 1, 256, 4)
+TT_LEGACY(mttkrp_32_eb_pr_256_64_8, 1, 256, 8)
+TT_LEGACY(mttkrp_32_eb_pr_256_64_16, 1, 256, 16)
+TT_LEGACY(mttkrp_32_eb_pr_256_64_32, 1, 256, 32)
+TT_LEGACY(mttkrp_32_eb_pr_256_128_4, 1, 256, 4)
+TT_LEGACY(mttkrp_32_eb_pr_256_128_8, 1, 256, 8)
+TT_LEGACY(mttkrp_32_eb_pr_256_128_16, 1, 256, 16)
+TT_LEGACY(mttkrp_32_eb_pr_256_128_32, 1, 256, 32)
+TT_LEGACY(mttkrp_32_eb_pr_256_256_4, 1, 256, 4)
+TT_LEGACY(mttkrp_32_eb_pr_256_256_8, 1, 256, 8)
+TT_LEGACY(mttkrp_32_eb_pr_256_256_16, 1, 256, 16)
+TT_LEGACY(mttkrp_32_eb_pr_256_256_32, 1, 256, 32)
+TT_LEGACY(mttkrp_32_eb_pr_512_64_4, 1, 512, 4)
+TT_LEGACY(mttkrp_32_eb_pr_512_64_8, 1, 512, 8)
+TT_LEGACY(mttkrp_32_eb_pr_512_64_16, 1, 512, 16)
+TT_LEGACY(mttkrp_32_eb_pr_512_64_32, 1, 512, 32)
+TT_LEGACY(mttkrp_32_eb_pr_512_128_4, 1, 512, 4)
+TT_LEGACY(mttkrp_32_eb_pr_512_128_8, 1, 512, 8)
+TT_LEGACY(mttkrp_32_eb_pr_512_128_16, 1, 512, 16)
+TT_LEGACY(mttkrp_32_eb_pr_512_128_32, 1, 512, 32)
+TT_LEGACY(mttkrp_32_eb_pr_512_256_4, 1, 512, 4)
+TT_LEGACY(mttkrp_32_eb_pr_512_256_8, 1, 512, 8)
+TT_LEGACY(mttkrp_32_eb_pr_512_256_16, 1, 512, 16)
+TT_LEGACY(mttkrp_32_eb_pr_512_256_32, 1, 512, 32)
+TT_LEGACY(mttkrp_32_eb_pr_512_512_4, 1, 512, 4)
+TT_LEGACY(mttkrp_32_eb_pr_512_512_8, 1, 512, 8)
+TT_LEGACY(mttkrp_32_eb_pr_512_512_16, 1, 512, 16)
+TT_LEGACY(mttkrp_32_eb_pr_512_512_32, 1, 512, 32)
+#undef TT_LEGACY
+
+int tentaco_num_variants(void) { return kNumVariants; }
+
+const char* tentaco_variant_name(int v) {
+  static std::vector<std::string> names = [] {
+    std::vector<std::string> n;
+    for (int i = 0; i < kNumVariants; ++i) {
+      const Variant vr = variant_of(i);
+      char buf[64];
+      if (vr.family == kSliceRow) std::snprintf(buf, sizeof buf, "sr_b%d", vr.block);
+      else
+        std::snprintf(buf, sizeof buf, "%s_b%d_g%d", vr.family == kEdgeSerial ? "es" : "ep",
+                      vr.block, vr.group);
+      n.emplace_back(buf);
+    }
+    return n;
+  }();
+  if (v < 0 || v >= kNumVariants) return nullptr;
+  return names[v].c_str();
+}
+
+int tentaco_variant_params(int v, int* family, int* block, int* group) {
+  const Variant vr = variant_of(v);
+  if (vr.family < 0) return fail(TENTACO_EINVAL, "variant %d out of range", v);
+  if (family) *family = vr.family;
+  if (block) *block = vr.block;
+  if (group) *group = vr.group;
+  return TENTACO_OK;
+}
+
+int tentaco_default_variant(void) { return default_variant(); }
+
+int tentaco_dev_malloc(void** ptr, size_t bytes) {
+  if (!ptr) return fail(TENTACO_EINVAL, "ptr is NULL");
+  if (hipMalloc(ptr, std::max<size_t>(bytes, 4)) != hipSuccess) {
+    *ptr = nullptr;
+    return fail(TENTACO_ENOMEM, "hipMalloc of %zu bytes failed", bytes);
+  }
+  return TENTACO_OK;
+}
+
+int tentaco_dev_free(void* ptr) {
+  TT_HIP_CHECK(hipFree(ptr));
+  return TENTACO_OK;
+}
+
+int tentaco_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+  TT_HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return TENTACO_OK;
+}
+
+int tentaco_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+  TT_HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return TENTACO_OK;
+}
+
+int tentaco_dev_synchronize(void) {
+  TT_HIP_CHECK(hipDeviceSynchronize());
+  return TENTACO_OK;
+}
+
+int tentaco_flush_cache(void* stream) {
+  // Writes a 512 MiB buffer (twice the 256 MB Infinity Cache) so that the
+  // next kernel starts with cold L2/MALL; the analog of Timer::clear_cache
+  // (timers.h:62-75) for the GPU timing protocol.
+  static void* buf = nullptr;
+  static std::mutex mu;
+  constexpr size_t kBytes = size_t(512) << 20;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    if (!buf && hipMalloc(&buf, kBytes) != hipSuccess) {
+      buf = nullptr;
+      return fail(TENTACO_ENOMEM, "flush buffer allocation failed");
+    }
+  }
+  return launch_fill(static_cast<float*>(buf), int64_t(kBytes / sizeof(float)), 1.0f,
+                     static_cast<hipStream_t>(stream));
+}
+
+int tentaco_timer_start(void* stream) {
+  std::lock_guard<std::mutex> lk(g_timer_mu);
+  int rc = ensure_events();
+  if (rc) return rc;
+  TT_HIP_CHECK(hipEventRecord(g_ev0, static_cast<hipStream_t>(stream)));
+  return TENTACO_OK;
+}
+
+int tentaco_timer_stop(void* stream, float* ms) {
+  std::lock_guard<std::mutex> lk(g_timer_mu);
+  int rc = ensure_events();
+  if (rc) return rc;
+  TT_HIP_CHECK(hipEventRecord(g_ev1, static_cast<hipStream_t>(stream)));
+  TT_HIP_CHECK(hipEventSynchronize(g_ev1));
+  float t = -1.f;
+  TT_HIP_CHECK(hipEventElapsedTime(&t, g_ev0, g_ev1));
+  if (ms) *ms = t;
+  return TENTACO_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,512 @@
+// Host-side CSF construction: COO -> CSF builder (the role splatt_csf_load
+// plays for the reference, ds.h:351-384, with the root kept dense as
+// compress_top_level does, ds.h:426-436) and the synthetic tensor generators
+// used for the benchmark configurations. Owned host CSFs are exposed through
+// taco_tensor_t views.
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <numeric>
+#include <thread>
+#include <utility>
+#include <vector>
+
+#include "internal.h"
+#include "host_csf.h"
+
+namespace tt {
+
+// ---------------------------------------------------------------------------
+// RNG: splitmix64 seeding + xoshiro256** (public-domain algorithms by
+// Steele/Lea/Flood and Blackman/Vigna).
+namespace {
+
+uint64_t splitmix64(uint64_t& x) {
+  uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+struct Rng {
+  uint64_t s[4];
+  Rng(uint64_t seed, uint64_t stream) {
+    uint64_t x = seed ^ (stream * 0xD1B54A32D192ED03ull) ^ 0x5851F42D4C957F2Dull;
+    for (auto& v : s) v = splitmix64(x);
+  }
+  static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+  uint64_t next() {
+    const uint64_t r = rotl(s[1] * 5, 7) * 9;
+    const uint64_t t = s[1] << 17;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = rotl(s[3], 45);
+    return r;
+  }
+  // uniform double in [0, 1) with 53 random bits
+  double uniform() { return double(next() >> 11) * 0x1.0p-53; }
+  // uniform integer in [0, n)
+  uint64_t below(uint64_t n) {
+    // Lemire's multiply-shift with rejection
+    uint64_t x = next();
+    __uint128_t m = __uint128_t(x) * n;
+    uint64_t l = uint64_t(m);
+    if (l < n) {
+      const uint64_t t = (0 - n) % n;
+      while (l < t) {
+        x = next();
+        m = __uint128_t(x) * n;
+        l = uint64_t(m);
+      }
+    }
+    return uint64_t(m >> 64);
+  }
+  double normal() {
+    // Box-Muller
+    double u1 = uniform();
+    while (u1 <= 0.0) u1 = uniform();
+    const double u2 = uniform();
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+  }
+  float value() { return float(uniform() * 2.0 - 1.0); }  // U[-1, 1)
+};
+
+// Zipf(s) on ranks 1..n through a cumulative table (exact for any s > 0).
+struct ZipfTable {
+  std::vector<double> cdf;
+  ZipfTable(int64_t n, double s) : cdf(size_t(n)) {
+    double acc = 0.0;
+    for (int64_t k = 1; k <= n; ++k) {
+      acc += std::pow(double(k), -s);
+      cdf[size_t(k - 1)] = acc;
+    }
+    for (auto& v : cdf) v /= acc;
+  }
+  // rank in 1..n
+  int64_t draw(Rng& r) const {
+    const double u = r.uniform();
+    const auto it = std::upper_bound(cdf.begin(), cdf.end(), u);
+    int64_t k = int64_t(it - cdf.begin()) + 1;
+    return std::min<int64_t>(k, int64_t(cdf.size()));
+  }
+};
+
+std::shared_ptr<const ZipfTable> zipf_table(int64_t n, double s) {
+  static std::mutex mu;
+  static std::map<std::pair<int64_t, int64_t>, std::shared_ptr<const ZipfTable>> cache;
+  const auto key = std::make_pair(n, int64_t(std::llround(s * 1e9)));
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  if (cache.size() > 64) cache.clear();
+  auto t = std::make_shared<const ZipfTable>(n, s);
+  cache.emplace(key, t);
+  return t;
+}
+
+// k distinct sorted integers from [0, n), k <= n.
+void distinct_sorted(Rng& r, int64_t k, int64_t n, int32_t* out) {
+  if (k <= 0) return;
+  if (k * 4 >= n) {
+    // selection sampling (Knuth, Algorithm S): output is already sorted
+    int64_t need = k;
+    for (int64_t i = 0; i < n && need > 0; ++i) {
+      if (r.below(uint64_t(n - i)) < uint64_t(need)) {
+        out[k - need] = int32_t(i);
+        --need;
+      }
+    }
+    return;
+  }
+  // Floyd's algorithm with a small open-addressing set, then sort
+  size_t cap = 16;
+  while (cap < size_t(k) * 2) cap <<= 1;
+  std::vector<int64_t> table(cap, -1);
+  auto insert = [&](int64_t v) {
+    size_t h = size_t((uint64_t(v) * 0x9E3779B97F4A7C15ull) >> 20) & (cap - 1);
+    while (table[h] >= 0) {
+      if (table[h] == v) return false;
+      h = (h + 1) & (cap - 1);
+    }
+    table[h] = v;
+    return true;
+  };
+  int64_t m = 0;
+  for (int64_t j = n - k; j < n; ++j) {
+    const int64_t t = int64_t(r.below(uint64_t(j + 1)));
+    if (insert(t)) out[m++] = int32_t(t);
+    else {
+      insert(j);
+      out[m++] = int32_t(j);
+    }
+  }
+  std::sort(out, out + k);
+}
+
+int64_t lognormal_count(Rng& r, double mu, double sigma, int64_t lo, int64_t hi) {
+  const double x = std::exp(mu + sigma * r.normal());
+  int64_t v = int64_t(std::llround(x));
+  if (v < lo) v = lo;
+  if (v > hi) v = hi;
+  return v;
+}
+
+int hw_threads() {
+  unsigned n = std::thread::hardware_concurrency();
+  if (n == 0) n = 1;
+  if (n > 32) n = 32;
+  return int(n);
+}
+
+template <class F>
+void parallel_for(int64_t n, F&& f) {
+  const int T = std::min<int64_t>(hw_threads(), std::max<int64_t>(n, 1));
+  if (T <= 1 || n < 2) {
+    for (int64_t i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::vector<std::thread> th;
+  std::atomic<int64_t> next{0};
+  for (int t = 0; t < T; ++t)
+    th.emplace_back([&] {
+      for (;;) {
+        const int64_t i = next.fetch_add(1);
+        if (i >= n) break;
+        f(i);
+      }
+    });
+  for (auto& x : th) x.join();
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+void HostCsf::make_view() {
+  dims_i32.assign(dims.begin(), dims.end());
+  idx_ptrs.assign(size_t(order) * 2, nullptr);
+  lvl_ptrs.assign(size_t(order), nullptr);
+  for (int m = 0; m < order; ++m) {
+    idx_ptrs[2 * m + 0] = pos[m].data();
+    idx_ptrs[2 * m + 1] = crd[m].data();
+    lvl_ptrs[m] = &idx_ptrs[2 * m];
+  }
+  std::memset(&view, 0, sizeof view);
+  view.order = order;
+  view.dimensions = dims_i32.data();
+  view.csize = 4;
+  view.indices = lvl_ptrs.data();
+  view.vals = vals.data();
+  view.vals_size = int32_t(vals.size());
+}
+
+namespace {
+
+// Lexicographic COO -> CSF. Stable ordering, duplicates summed in input order
+// (FP32), dense root (pos0 = {0, dims0}, crd0 = 0..dims0-1).
+int build_from_coo(int order, const int32_t* dims, int64_t n, const int32_t* coords,
+                   const float* vals, HostCsf* out) {
+  // total key bits; radix-sort a packed 64-bit key when the coordinates fit
+  int bits[kMaxOrder] = {};
+  int total_bits = 0;
+  for (int m = 0; m < order; ++m) {
+    int b = 1;
+    while ((int64_t(1) << b) < int64_t(dims[m])) ++b;
+    bits[m] = b;
+    total_bits += b;
+  }
+  std::vector<int64_t> perm(static_cast<size_t>(n));
+  std::iota(perm.begin(), perm.end(), 0);
+  if (total_bits <= 64) {
+    std::vector<uint64_t> key(static_cast<size_t>(n));
+    parallel_for((n + 65535) / 65536, [&](int64_t chunk) {
+      const int64_t a = chunk * 65536, e = std::min<int64_t>(n, a + 65536);
+      for (int64_t q = a; q < e; ++q) {
+        uint64_t k = 0;
+        for (int m = 0; m < order; ++m) k = (k << bits[m]) | uint64_t(coords[m * n + q]);
+        key[size_t(q)] = k;
+      }
+    });
+    // LSD radix sort, 16 bits per pass, stable
+    std::vector<int64_t> tmp(static_cast<size_t>(n));
+    std::vector<uint64_t> ktmp(static_cast<size_t>(n));
+    for (int shift = 0; shift < total_bits; shift += 16) {
+      std::vector<int64_t> cnt(65537, 0);
+      for (int64_t q = 0; q < n; ++q) ++cnt[((key[size_t(q)] >> shift) & 0xFFFF) + 1];
+      for (int i = 0; i < 65536; ++i) cnt[i + 1] += cnt[i];
+      for (int64_t q = 0; q < n; ++q) {
+        const size_t d = size_t(cnt[(key[size_t(q)] >> shift) & 0xFFFF]++);
+        tmp[d] = perm[size_t(q)];
+        ktmp[d] = key[size_t(q)];
+      }
+      perm.swap(tmp);
+      key.swap(ktmp);
+    }
+  } else {
+    std::stable_sort(perm.begin(), perm.end(), [&](int64_t a, int64_t b) {
+      for (int m = 0; m < order; ++m) {
+        const int32_t ca = coords[m * n + a], cb = coords[m * n + b];
+        if (ca != cb) return ca < cb;
+      }
+      return false;
+    });
+  }
+  HostCsf& h = *out;
+  h.order = order;
+  h.dims.assign(dims, dims + order);
+  h.pos.assign(size_t(order), {});
+  h.crd.assign(size_t(order), {});
+  h.pos[0] = {0, dims[0]};
+  h.crd[0].resize(size_t(dims[0]));
+  std::iota(h.crd[0].begin(), h.crd[0].end(), 0);
+  h.vals.clear();
+  h.vals.reserve(size_t(n));
+  for (int m = 1; m < order; ++m) h.crd[m].reserve(size_t(n));
+  // level-1 nodes per root coordinate (dense root), turned into pos[1] below
+  std::vector<int32_t> per_root(size_t(dims[0]) + 1, 0);
+  const int32_t* prev = nullptr;
+  int64_t prev_p = -1;
+  for (int64_t q = 0; q < n; ++q) {
+    const int64_t p = perm[size_t(q)];
+    // smallest level whose coordinate differs from the previous nonzero
+    int d = 0;
+    if (prev_p >= 0) {
+      d = order;
+      for (int m = 0; m < order; ++m)
+        if (coords[m * n + p] != coords[m * n + prev_p]) {
+          d = m;
+          break;
+        }
+    }
+    if (d == order) {
+      h.vals.back() += vals[p];  // duplicate coordinate: summed in input order
+      continue;
+    }
+    for (int m = std::max(d, 1); m < order; ++m) {
+      if (m == 1) ++per_root[size_t(coords[p]) + 1];
+      // a new level-m node; its children start at the current size of level m+1
+      h.crd[m].push_back(coords[m * n + p]);
+      if (m + 1 < order) h.pos[m + 1].push_back(int32_t(h.crd[m + 1].size()));
+    }
+    h.vals.push_back(vals[p]);
+    prev_p = p;
+  }
+  (void)prev;
+  h.pos[1].resize(size_t(dims[0]) + 1);
+  int32_t run = 0;
+  for (int32_t i = 0; i <= dims[0]; ++i) {
+    run += per_root[size_t(i)];
+    h.pos[1][size_t(i)] = run;
+  }
+  for (int m = 2; m < order; ++m) h.pos[m].push_back(int32_t(h.crd[m].size()));
+  h.make_view();
+  return TENTACO_OK;
+}
+
+// Synthetic CSF: slice ids Zipf(slice_s) over the root dimension (distinct
+// draws until the nonzero target is met), level-1 fan-out Zipf(fanout_s)
+// over that level's dimension, fan-out of levels 2..N-2 round(lognormal(mid))
+// and leaf fiber length round(lognormal(leaf)) capped at leaf_cap; children
+// coordinates distinct, sorted, uniform; values U[-1, 1). The last fiber is
+// truncated so the tensor has exactly min(nnz, reachable) nonzeros.
+int synth(const tentaco_synth_params& P, HostCsf* out) {
+  const int N = P.order;
+  const int64_t target = P.nnz;
+  std::vector<int64_t> dims(static_cast<size_t>(N));
+  for (int m = 0; m < N; ++m) dims[size_t(m)] = P.dims[m];
+  auto zs = zipf_table(dims[0], P.slice_s);
+  std::shared_ptr<const ZipfTable> zf;
+  if (N >= 3) zf = zipf_table(dims[1], P.fanout_s);
+  const int64_t leaf_hi = std::min<int64_t>(P.leaf_cap > 0 ? P.leaf_cap : dims[size_t(N - 1)],
+                                            dims[size_t(N - 1)]);
+  // ---- phase 1: slice draw order and tree shapes (children counts per level)
+  struct Shape {
+    int32_t id;
+    std::vector<std::vector<int32_t>> cnt;  // cnt[m]: children counts of level m-1 nodes
+    int64_t nnz;
+  };
+  std::vector<Shape> slices;
+  std::vector<uint8_t> used(size_t(dims[0]), 0);
+  Rng r0(P.seed, 0);
+  int64_t total = 0, nused = 0, draws = 0;
+  const int64_t max_draws = std::max<int64_t>(int64_t(100) * dims[0], 1000000);
+  while (total < target && nused < dims[0] && draws < max_draws) {
+    const int64_t id = zs->draw(r0) - 1;
+    ++draws;
+    if (used[size_t(id)]) continue;
+    used[size_t(id)] = 1;
+    ++nused;
+    Rng rs(P.seed, uint64_t(1) + uint64_t(id));
+    Shape sh;
+    sh.id = int32_t(id);
+    sh.cnt.assign(size_t(N), {});
+    // level 1 fan-out of the slice
+    int64_t n1;
+    if (N == 2) n1 = lognormal_count(rs, P.leaf_mu, P.leaf_sigma, 1, leaf_hi);
+    else n1 = zf->draw(rs);
+    sh.cnt[1].push_back(int32_t(n1));
+    int64_t nodes = n1;
+    for (int m = 2; m < N; ++m) {
+      sh.cnt[size_t(m)].resize(size_t(nodes));
+      int64_t next = 0;
+      for (int64_t x = 0; x < nodes; ++x) {
+        int64_t c;
+        if (m == N - 1) c = lognormal_count(rs, P.leaf_mu, P.leaf_sigma, 1, leaf_hi);
+        else c = lognormal_count(rs, P.mid_mu, P.mid_sigma, 1, dims[size_t(m)]);
+        sh.cnt[size_t(m)][size_t(x)] = int32_t(c);
+        next += c;
+      }
+      nodes = next;
+    }
+    sh.nnz = nodes;
+    if (total + sh.nnz > target) {
+      // truncate: keep a prefix of the leaves, then trim every level above
+      int64_t keep = target - total;
+      for (int m = N - 1; m >= 1; --m) {
+        auto& c = sh.cnt[size_t(m)];
+        int64_t acc = 0;
+        size_t k = 0;
+        for (; k < c.size(); ++k) {
+          if (acc + c[k] >= keep) {
+            c[k] = int32_t(keep - acc);
+            ++k;
+            break;
+          }
+          acc += c[k];
+        }
+        c.resize(k);
+        keep = int64_t(k);  // number of parents (level m-1 nodes) kept
+      }
+      sh.nnz = target - total;
+    }
+    total += sh.nnz;
+    slices.push_back(std::move(sh));
+  }
+  std::sort(slices.begin(), slices.end(),
+            [](const Shape& a, const Shape& b) { return a.id < b.id; });
+  // ---- phase 2: global pos arrays, then coordinates level by level
+  HostCsf& h = *out;
+  h.order = N;
+  h.dims.assign(P.dims, P.dims + N);
+  h.pos.assign(size_t(N), {});
+  h.crd.assign(size_t(N), {});
+  h.pos[0] = {0, P.dims[0]};
+  h.crd[0].resize(size_t(dims[0]));
+  std::iota(h.crd[0].begin(), h.crd[0].end(), 0);
+  h.pos[1].assign(size_t(dims[0]) + 1, 0);
+  for (const Shape& sh : slices) h.pos[1][size_t(sh.id) + 1] = sh.cnt[1][0];
+  for (size_t i = 0; i < size_t(dims[0]); ++i) h.pos[1][i + 1] += h.pos[1][i];
+  for (int m = 2; m < N; ++m) {
+    auto& p = h.pos[size_t(m)];
+    p.push_back(0);
+    for (const Shape& sh : slices)
+      for (int32_t c : sh.cnt[size_t(m)]) p.push_back(p.back() + c);
+  }
+  for (int m = 1; m < N; ++m) {
+    const std::vector<int32_t>& pp = h.pos[size_t(m)];
+    const int64_t parents = int64_t(pp.size()) - 1;
+    h.crd[size_t(m)].resize(size_t(pp.back()));
+    int32_t* crd = h.crd[size_t(m)].data();
+    const int64_t dim = dims[size_t(m)];
+    const int64_t chunks = (parents + 1023) / 1024;
+    parallel_for(chunks, [&](int64_t ch) {
+      const int64_t a = ch * 1024, e = std::min<int64_t>(parents, a + 1024);
+      for (int64_t x = a; x < e; ++x) {
+        const int64_t k = pp[size_t(x + 1)] - pp[size_t(x)];
+        if (k <= 0) continue;
+        Rng rc(P.seed, (uint64_t(m) << 56) ^ (uint64_t(1) << 55) ^ uint64_t(x));
+        distinct_sorted(rc, k, dim, crd + pp[size_t(x)]);
+      }
+    });
+  }
+  h.vals.resize(size_t(total));
+  const int64_t vchunks = (total + 65535) / 65536;
+  parallel_for(vchunks, [&](int64_t ch) {
+    Rng rv(P.seed, (uint64_t(7) << 56) ^ uint64_t(ch));
+    const int64_t a = ch * 65536, e = std::min<int64_t>(total, a + 65536);
+    for (int64_t q = a; q < e; ++q) h.vals[size_t(q)] = rv.value();
+  });
+  h.make_view();
+  return TENTACO_OK;
+}
+
+}  // namespace
+}  // namespace tt
+
+using tt::fail;
+
+extern "C" {
+
+int tentaco_coo_to_csf(int order, const int32_t* dims, int64_t nnz, const int32_t* coords,
+                       const float* vals, tentaco_host_csf** out) {
+  tt::clear_error();
+  if (!out) return fail(TENTACO_EINVAL, "out is NULL");
+  *out = nullptr;
+  if (order < 2 || order > tt::kMaxOrder)
+    return fail(TENTACO_EUNSUPPORTED, "order %d outside 2..5", order);
+  if (!dims || nnz < 0 || (nnz > 0 && (!coords || !vals)))
+    return fail(TENTACO_EINVAL, "NULL dims/coords/vals");
+  if (nnz >= (int64_t(1) << 31) - (int64_t(1) << 16))
+    return fail(TENTACO_EUNSUPPORTED, "nnz %lld beyond the int32 index limit", (long long)nnz);
+  for (int m = 0; m < order; ++m)
+    if (dims[m] <= 0) return fail(TENTACO_EINVAL, "dims[%d] = %d", m, dims[m]);
+  for (int m = 0; m < order; ++m)
+    for (int64_t q = 0; q < nnz; ++q) {
+      const int32_t c = coords[m * nnz + q];
+      if (c < 0 || c >= dims[m])
+        return fail(TENTACO_EINVAL, "coordinate %d of nonzero %lld is %d, outside [0,%d)", m,
+                    (long long)q, c, dims[m]);
+    }
+  auto* h = new tentaco_host_csf;
+  const int rc = tt::build_from_coo(order, dims, nnz, coords, vals, &h->h);
+  if (rc) {
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return TENTACO_OK;
+}
+
+int tentaco_synth_csf(const tentaco_synth_params* p, tentaco_host_csf** out) {
+  tt::clear_error();
+  if (!p || !out) return fail(TENTACO_EINVAL, "NULL argument");
+  *out = nullptr;
+  if (p->order < 2 || p->order > tt::kMaxOrder)
+    return fail(TENTACO_EUNSUPPORTED, "order %d outside 2..5", p->order);
+  for (int m = 0; m < p->order; ++m)
+    if (p->dims[m] <= 0) return fail(TENTACO_EINVAL, "dims[%d] = %d", m, p->dims[m]);
+  if (p->nnz < 0 || p->nnz >= (int64_t(1) << 31) - (int64_t(1) << 16))
+    return fail(TENTACO_EUNSUPPORTED, "nnz %lld outside the int32 index range", (long long)p->nnz);
+  if (p->slice_s <= 0 || (p->order >= 3 && p->fanout_s <= 0))
+    return fail(TENTACO_EINVAL, "Zipf exponents must be > 0");
+  auto* h = new tentaco_host_csf;
+  const int rc = tt::synth(*p, &h->h);
+  if (rc) {
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return TENTACO_OK;
+}
+
+taco_tensor_t* tentaco_host_csf_view(tentaco_host_csf* h) { return h ? &h->h.view : nullptr; }
+
+int64_t tentaco_host_csf_nodes(const tentaco_host_csf* h, int level) {
+  if (!h || level < 0 || level >= h->h.order) return -1;
+  return int64_t(h->h.crd[size_t(level)].size());
+}
+
+int tentaco_host_csf_free(tentaco_host_csf* h) {
+  delete h;
+  return TENTACO_OK;
+}
+
+}  // extern "C"

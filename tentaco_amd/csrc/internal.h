@@ -1,0 +1,89 @@
+// Internal declarations shared by the tentaco HIP sources (not installed).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+
+#include "../../include/taco_tensor_t.h"
+#include "../../include/tentaco.h"
+
+namespace tt {
+
+constexpr int kMaxOrder = 5;
+
+// Device view of a CSF tensor. Level 0 is the root (n0 nodes, output row of
+// node x is crd0[x]); level m >= 1 has nodes[m] nodes whose parent ranges are
+// pos[m][x]..pos[m][x+1] for parent x at level m-1; level order-1 holds the
+// nonzeros (nodes[order-1] == nnz) with their values in vals.
+struct CsfDev {
+  int order = 0;
+  int32_t nnz = 0;
+  int32_t nodes[kMaxOrder] = {};
+  const int32_t* crd0 = nullptr;
+  const int32_t* pos[kMaxOrder] = {};
+  const int32_t* crd[kMaxOrder] = {};
+  const float* vals = nullptr;
+};
+
+// Factor matrices by level: M[m] for m = 1..order-1, row-major rows x R.
+struct FactorsDev {
+  const float* M[kMaxOrder] = {};
+};
+
+// Kernel families of the MTTKRP variant table.
+enum Family { kEdgeSerial = 0, kEdgeParallel = 1, kSliceRow = 2 };
+
+struct Variant {
+  int family;
+  int block;  // threads per workgroup
+  int group;  // nonzeros per lane group (edge families); 0 for slice-row
+};
+
+constexpr int kNumGridBlocks = 3;
+constexpr int kNumGridGroups = 5;
+constexpr int kGridBlocks[kNumGridBlocks] = {128, 256, 512};
+constexpr int kGridGroups[kNumGridGroups] = {4, 8, 16, 32, 64};
+// 2 edge families x 3 blocks x 5 groups, then 3 slice-row block sizes.
+constexpr int kNumVariants = 2 * kNumGridBlocks * kNumGridGroups + kNumGridBlocks;
+
+Variant variant_of(int v);
+int default_variant();
+
+// Lanes that share one factor row for rank R (float4 per lane); 16 for the
+// generic (non-specialised) rank path.
+int lanes_per_row(int R);
+// Nonzeros one workgroup covers for an edge-family variant at rank R.
+int64_t nnz_per_block(const Variant& vr, int R);
+// Workspace bytes an MTTKRP call needs (0 for the slice-row family).
+size_t mttkrp_workspace_bytes(const CsfDev& B, int R, const Variant& vr);
+size_t ttv_workspace_bytes(const CsfDev& B);
+
+// Enqueue one MTTKRP on `stream`. A (a_rows x R) is fully overwritten;
+// `dense_root` says crd0 == 0..n0-1 with n0 == a_rows (no zero fill needed
+// for the slice-row family). ws must hold mttkrp_workspace_bytes bytes.
+int launch_mttkrp(const CsfDev& B, const FactorsDev& F, int R, float* A,
+                  int32_t a_rows, bool dense_root, const Variant& vr,
+                  void* ws, hipStream_t stream);
+// Enqueue one TTV: y has nodes[order-2] floats, c the last mode's extent.
+int launch_ttv(const CsfDev& B, const float* c, float* y, void* ws,
+               hipStream_t stream);
+
+// Fill n floats with value on `stream` (kernel, not hipMemsetAsync, so it is
+// safe inside captured graphs).
+int launch_fill(float* p, int64_t n, float value, hipStream_t stream);
+
+// Error plumbing: record a message for tentaco_last_error() and return code.
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+void clear_error();
+
+}  // namespace tt
+
+#define TT_HIP_CHECK(expr)                                                  \
+  do {                                                                      \
+    hipError_t tt_e_ = (expr);                                              \
+    if (tt_e_ != hipSuccess)                                                \
+      return ::tt::fail(TENTACO_EHIP, "%s failed: %s (%s:%d)", #expr,       \
+                        hipGetErrorString(tt_e_), __FILE__, __LINE__);      \
+  } while (0)

@@ -1,0 +1,528 @@
+// CSF MTTKRP kernels for gfx950 (wave64), orders 2..5.
+//
+//   A(i,:) = sum over the subtree of root node i of
+//            B(i,k,l,...) * C(k,:) (.) D(l,:) (.) ...
+//
+// computed in factorised form (the association of
+// tensor_kernels/mttkrp{3,4}_csf_cpu_taco_workspace.h and of
+// mttkrp5_csf_cpu_taco.h): the nonzeros of a level-(N-2) fiber are summed as
+// val * M_{N-1}(l,:), the fiber sum is multiplied by its own factor row once,
+// and so on up to the root. Factor rows are gathered as float4 fragments
+// over a row group of lanes (rowops.h).
+//
+// Three kernel families form the variant table:
+//  * edge-serial (ES): the nonzeros are cut into equal runs of G consecutive
+//    positions, one run per row group (the reference's eb_sr work split,
+//    mttkrp3_csf_ebsr.h:4-77). A run walks its positions serially, closing
+//    fibers and slices as it crosses their ends. Slices that start and end
+//    inside one run are stored; a run's first and last slice may be shared
+//    with neighbouring runs and are added atomically into the pre-zeroed A.
+//  * edge-parallel (EP): the same runs, but the shared first/last slice
+//    partials of all runs of a workgroup are combined by a segmented
+//    parallel reduction in LDS (the role of segReduceGroup,
+//    gpu_library.h:295-321, redone for wave64 and R-wide rows); only the
+//    workgroup's first and last slice are added atomically.
+//  * slice-row (SR): one wavefront per root node, its level-1 children
+//    spread over the row groups, rows reduced across the wave with xor
+//    shuffles and stored once. No atomics, no zero fill for a dense root,
+//    deterministic; balanced only when slices are.
+//
+// Runs find their starting node at every level with an upper-bound binary
+// search inside the workgroup's bracket, which a prep kernel computes
+// together with the zero fill of A (the reference's two partition-search
+// launches plus cudaMemset, mttkrp3_csf_gpu_taco.h:88-94, in one launch).
+
+#include <algorithm>
+#include <climits>
+
+#include "internal.h"
+#include "rowops.h"
+#include "tuning.h"
+
+namespace tt {
+namespace {
+
+// ---------------------------------------------------------------------------
+// prep: zero A and compute, for every workgroup boundary b (0..nblk), the
+// node at each level that contains nonzero position min(b*NB, nnz-1).
+// brk has (nblk+1) x (N-1) entries, level-major per boundary.
+__global__ __launch_bounds__(256) void edge_prep_kernel(CsfDev B, float* A,
+                                                        int64_t a_elems,
+                                                        int32_t* brk,
+                                                        int32_t nblk,
+                                                        int32_t NB) {
+  const int64_t tid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t nthr = int64_t(gridDim.x) * blockDim.x;
+  // zero fill, float4 body + scalar tail
+  const int64_t n4 = a_elems >> 2;
+  float4* A4 = reinterpret_cast<float4*>(A);
+  for (int64_t i = tid; i < n4; i += nthr) A4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int64_t i = (n4 << 2) + tid; i < a_elems; i += nthr) A[i] = 0.f;
+  if (B.nnz <= 0) return;
+  const int N = B.order;
+  for (int64_t b = tid; b <= nblk; b += nthr) {
+    int64_t p = b * int64_t(NB);
+    if (p > B.nnz - 1) p = B.nnz - 1;
+    int32_t key = int32_t(p);
+    for (int m = N - 2; m >= 0; --m) {
+      const int32_t x = search_le(B.pos[m + 1], 0, B.nodes[m] - 1, key);
+      brk[b * (N - 1) + m] = x;
+      key = x;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// edge-serial / edge-parallel kernel.
+template <int N, int R, bool EP>
+__global__ __launch_bounds__(512) void edge_kernel(CsfDev B, FactorsDev F,
+                                                   float* __restrict__ A,
+                                                   int Rr,
+                                                   const int32_t* __restrict__ brk,
+                                                   int32_t G, int32_t NB) {
+  using RO = Row<R>;
+  using V = typename RO::V;
+  constexpr int NV = RO::NV;
+  constexpr int LPR = RO::LPR;
+  constexpr int NL = N - 1;  // levels 0..N-2 carry walk state
+
+  const int gi = threadIdx.x / LPR;
+  const int c = threadIdx.x % LPR;
+  const int32_t b = blockIdx.x;
+  const int32_t nnz = B.nnz;
+  const int32_t blk0 = b * NB;
+  const int32_t blk1 = min(blk0 + NB, nnz);
+  const int32_t p0 = blk0 + gi * G;
+  const int32_t p1 = min(p0 + G, blk1);
+  const bool active = p0 < p1;
+
+  int32_t cur[NL], cend[NL], hib[NL];
+  V acc[NL][NV];
+  V rowc[NL][NV];
+#pragma unroll
+  for (int k = 0; k < NL; ++k) {
+    RO::zero(acc[k]);
+    RO::zero(rowc[k]);
+    cur[k] = 0;
+    cend[k] = 0;
+    hib[k] = 0;
+  }
+  int32_t head = INT_MAX;
+  bool head_closed = false;
+  V head_acc[NV];
+  RO::zero(head_acc);
+
+  if (active) {
+    // starting node at each level: bottom-up upper-bound searches inside the
+    // workgroup bracket
+    int32_t key = p0;
+#pragma unroll
+    for (int k = NL - 1; k >= 0; --k) {
+      const int32_t lo = brk[b * NL + k];
+      hib[k] = brk[(b + 1) * NL + k];
+      cur[k] = search_le(B.pos[k + 1], lo, hib[k], key);
+      key = cur[k];
+    }
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+      cend[k] = B.pos[k + 1][cur[k] + 1];
+      if (k >= 1) RO::load(rowc[k], F.M[k], B.crd[k][cur[k]], c, Rr);
+    }
+    head = cur[0];
+
+    const int32_t* __restrict__ lcrd = B.crd[N - 1];
+    const float* __restrict__ vals = B.vals;
+    const float* __restrict__ Mleaf = F.M[N - 1];
+    constexpr int U = 4;
+    for (int32_t t0 = p0; t0 < p1; t0 += U) {
+      int32_t l[U];
+      float v[U];
+      V d[U][NV];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int32_t t = t0 + u;
+        const bool ok = t < p1;
+        l[u] = ok ? lcrd[t] : 0;
+        v[u] = ok ? vals[t] : 0.f;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) RO::load(d[u], Mleaf, l[u], c, Rr);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int32_t t = t0 + u;
+        if (t >= p1) break;
+        if (t >= cend[NL - 1]) {
+          // close the fiber at level N-2 and every ancestor that ends here
+          bool closing = true;
+          int32_t ckey = t;
+#pragma unroll
+          for (int k = NL - 1; k >= 0; --k) {
+            if (closing) {
+              const int32_t old = cur[k];
+              if (k >= 1) {
+                RO::hfma(acc[k - 1], acc[k], rowc[k]);
+              } else {
+                const int64_t row = B.crd0[old];
+                if (old == head && !head_closed) {
+                  head_closed = true;
+                  if constexpr (EP) {
+#pragma unroll
+                    for (int j = 0; j < NV; ++j) head_acc[j] = acc[0][j];
+                  } else {
+                    RO::atomic_add(A, row, c, Rr, acc[0]);
+                  }
+                } else {
+                  RO::store(A, row, c, Rr, acc[0]);
+                }
+              }
+              RO::zero(acc[k]);
+              int32_t x = old + 1;
+              int32_t e = B.pos[k + 1][x + 1];
+              if (e <= ckey) {
+                x = search_le(B.pos[k + 1], x + 1, hib[k], ckey);
+                e = B.pos[k + 1][x + 1];
+              }
+              cur[k] = x;
+              cend[k] = e;
+              if (k >= 1) {
+                RO::load(rowc[k], F.M[k], B.crd[k][x], c, Rr);
+                closing = x >= cend[k - 1];
+                ckey = x;
+              }
+            }
+          }
+        }
+        RO::axpy(acc[NL - 1], v[u], d[u]);
+      }
+    }
+    // fold the open nodes into the open slice
+#pragma unroll
+    for (int k = NL - 1; k >= 1; --k) RO::hfma(acc[k - 1], acc[k], rowc[k]);
+    if constexpr (!EP) RO::atomic_add(A, B.crd0[cur[0]], c, Rr, acc[0]);
+  }
+
+  if constexpr (EP) {
+    // Segmented suffix scan over the 2 entries per row group (first slice,
+    // last slice) of the workgroup; keys are nondecreasing, idle groups use
+    // INT_MAX. Entry 2g = (head slice, its closed partial or 0), entry
+    // 2g+1 = (open slice at run end, its partial).
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int NGB = blockDim.x / LPR;
+    const int E = 2 * NGB;
+    const int Rc = (R == 0) ? Rr : R;
+    int32_t* keys = reinterpret_cast<int32_t*>(smem);
+    float* ev = reinterpret_cast<float*>(smem + size_t(E) * sizeof(int32_t));
+    V val[2][NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      val[0][j] = head_acc[j];
+      val[1][j] = acc[0][j];
+    }
+    const int32_t key0 = active ? head : INT_MAX;
+    const int32_t key1 = active ? cur[0] : INT_MAX;
+    if (c == 0) {
+      keys[2 * gi] = key0;
+      keys[2 * gi + 1] = key1;
+    }
+    auto put = [&](int e, const V (&x)[NV]) {
+#pragma unroll
+      for (int j = 0; j < NV; ++j) {
+        const int k = RO::col(j, c);
+        if constexpr (R == 0) {
+          if (k < Rc) ev[size_t(e) * Rc + k] = x[j];
+        } else {
+          *reinterpret_cast<float4*>(ev + size_t(e) * Rc + k) = x[j];
+        }
+      }
+    };
+    auto get = [&](int e, V (&x)[NV]) {
+#pragma unroll
+      for (int j = 0; j < NV; ++j) {
+        const int k = RO::col(j, c);
+        if constexpr (R == 0) {
+          x[j] = (k < Rc) ? ev[size_t(e) * Rc + k] : 0.f;
+        } else {
+          x[j] = *reinterpret_cast<const float4*>(ev + size_t(e) * Rc + k);
+        }
+      }
+    };
+    put(2 * gi, val[0]);
+    put(2 * gi + 1, val[1]);
+    __syncthreads();
+    for (int s = 1; s < E; s <<= 1) {
+      V add[2][NV];
+      bool take[2];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int e = 2 * gi + q;
+        take[q] = (e + s < E) && keys[e] != INT_MAX && keys[e + s] == keys[e];
+        if (take[q]) get(e + s, add[q]);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        if (take[q]) {
+          RO::add(val[q], add[q]);
+          put(2 * gi + q, val[q]);
+        }
+      }
+      __syncthreads();
+    }
+    // segment heads write their totals; only the workgroup's first and last
+    // slice can be shared with another workgroup
+    const int32_t g_last = (blk1 - 1 - blk0) / G;
+    const int32_t first_key = keys[0];
+    const int32_t last_key = keys[2 * g_last + 1];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int e = 2 * gi + q;
+      const int32_t k = keys[e];
+      if (k == INT_MAX) continue;
+      if (e > 0 && keys[e - 1] == k) continue;
+      const int64_t row = B.crd0[k];
+      if (k == first_key || k == last_key) RO::atomic_add(A, row, c, Rr, val[q]);
+      else RO::store(A, row, c, Rr, val[q]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// slice-row kernel: subtree sums, depth first.
+template <int N, int M, int R>
+__device__ __forceinline__ void subtree(const CsfDev& B, const FactorsDev& F,
+                                        int32_t y, int c, int Rr,
+                                        typename Row<R>::V (&out)[Row<R>::NV]) {
+  using RO = Row<R>;
+  using V = typename RO::V;
+  constexpr int NV = RO::NV;
+  if constexpr (M == N - 1) {
+    // a lone nonzero (order-2 tensors only reach this at level 1)
+    V d[NV];
+    RO::load(d, F.M[M], B.crd[M][y], c, Rr);
+    RO::zero(out);
+    RO::axpy(out, B.vals[y], d);
+  } else {
+    V own[NV];
+    RO::load(own, F.M[M], B.crd[M][y], c, Rr);
+    RO::zero(out);
+    const int32_t a = B.pos[M + 1][y];
+    const int32_t e = B.pos[M + 1][y + 1];
+    if constexpr (M + 1 == N - 1) {
+      const int32_t* __restrict__ lcrd = B.crd[N - 1];
+      const float* __restrict__ vals = B.vals;
+      const float* __restrict__ Ml = F.M[N - 1];
+      constexpr int U = 4;
+      for (int32_t q0 = a; q0 < e; q0 += U) {
+        int32_t l[U];
+        float v[U];
+        V d[U][NV];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const bool ok = q0 + u < e;
+          l[u] = ok ? lcrd[q0 + u] : 0;
+          v[u] = ok ? vals[q0 + u] : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) RO::load(d[u], Ml, l[u], c, Rr);
+#pragma unroll
+        for (int u = 0; u < U; ++u) RO::axpy(out, v[u], d[u]);
+      }
+    } else {
+      for (int32_t ch = a; ch < e; ++ch) {
+        V s[NV];
+        subtree<N, M + 1, R>(B, F, ch, c, Rr, s);
+        RO::add(out, s);
+      }
+    }
+    RO::hmul(out, own);
+  }
+}
+
+template <int N, int R>
+__global__ __launch_bounds__(512) void slice_kernel(CsfDev B, FactorsDev F,
+                                                    float* __restrict__ A,
+                                                    int Rr) {
+  using RO = Row<R>;
+  using V = typename RO::V;
+  constexpr int NV = RO::NV;
+  constexpr int LPR = RO::LPR;
+  constexpr int NG = 64 / LPR;
+  const int64_t wave = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+  if (wave >= B.nodes[0]) return;  // whole wavefront exits together
+  const int lane = threadIdx.x & 63;
+  const int gi = lane / LPR;
+  const int c = lane % LPR;
+  const int32_t x = int32_t(wave);
+  V acc[NV];
+  RO::zero(acc);
+  const int32_t a = B.pos[1][x];
+  const int32_t e = B.pos[1][x + 1];
+  if constexpr (N == 2) {
+    // children are nonzeros: row groups take them round-robin
+    for (int32_t y = a + gi; y < e; y += NG) {
+      V d[NV];
+      RO::load(d, F.M[1], B.crd[1][y], c, Rr);
+      RO::axpy(acc, B.vals[y], d);
+    }
+  } else {
+    for (int32_t y = a + gi; y < e; y += NG) {
+      V s[NV];
+      subtree<N, 1, R>(B, F, y, c, Rr, s);
+      RO::add(acc, s);
+    }
+  }
+  RO::wave_sum(acc);
+  if (gi == 0) RO::store(A, B.crd0[x], c, Rr, acc);
+}
+
+__global__ __launch_bounds__(256) void fill_kernel(float* A, int64_t n, float value) {
+  const int64_t tid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t nthr = int64_t(gridDim.x) * blockDim.x;
+  const int64_t n4 = n >> 2;
+  float4* A4 = reinterpret_cast<float4*>(A);
+  const float4 v4 = make_float4(value, value, value, value);
+  for (int64_t i = tid; i < n4; i += nthr) A4[i] = v4;
+  for (int64_t i = (n4 << 2) + tid; i < n; i += nthr) A[i] = value;
+}
+
+int grid_for(int64_t work, int block, int cap) {
+  int64_t g = (work + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return int(g);
+}
+
+template <int N, int R>
+int launch_typed(const CsfDev& B, const FactorsDev& F, int Rr, float* A,
+                 int32_t a_rows, bool dense_root, const Variant& vr, void* ws,
+                 hipStream_t st) {
+  const int64_t a_elems = int64_t(a_rows) * Rr;
+  if (vr.family == kSliceRow) {
+    if (!dense_root) {
+      fill_kernel<<<grid_for(a_elems / 4 + 1, 256, 8192), 256, 0, st>>>(A, a_elems, 0.f);
+      TT_HIP_CHECK(hipGetLastError());
+    }
+    if (B.nodes[0] > 0) {
+      const int64_t threads = int64_t(B.nodes[0]) * 64;
+      const int grid = int((threads + vr.block - 1) / vr.block);
+      slice_kernel<N, R><<<grid, vr.block, 0, st>>>(B, F, A, Rr);
+      TT_HIP_CHECK(hipGetLastError());
+    }
+    return TENTACO_OK;
+  }
+  const int64_t NB = nnz_per_block(vr, Rr);
+  const int64_t nblk = (int64_t(B.nnz) + NB - 1) / NB;
+  int32_t* brk = static_cast<int32_t*>(ws);
+  {
+    const int64_t work = std::max<int64_t>(a_elems / 4, nblk + 1);
+    edge_prep_kernel<<<grid_for(work, 256, 16384), 256, 0, st>>>(
+        B, A, a_elems, brk, int32_t(nblk), int32_t(NB));
+    TT_HIP_CHECK(hipGetLastError());
+  }
+  if (nblk == 0) return TENTACO_OK;
+  const int Rc = Rr;
+  if (vr.family == kEdgeSerial) {
+    edge_kernel<N, R, false><<<int(nblk), vr.block, 0, st>>>(
+        B, F, A, Rr, brk, vr.group, int32_t(NB));
+  } else {
+    const int lpr = Row<R>::LPR;
+    const int E = 2 * (vr.block / lpr);
+    const size_t lds = size_t(E) * sizeof(int32_t) + size_t(E) * Rc * sizeof(float);
+    if (lds > 65536)
+      return fail(TENTACO_EUNSUPPORTED, "edge-parallel block %d at rank %d needs %zu B of LDS",
+                  vr.block, Rc, lds);
+    edge_kernel<N, R, true><<<int(nblk), vr.block, lds, st>>>(
+        B, F, A, Rr, brk, vr.group, int32_t(NB));
+  }
+  TT_HIP_CHECK(hipGetLastError());
+  return TENTACO_OK;
+}
+
+template <int N>
+int launch_order(const CsfDev& B, const FactorsDev& F, int R, float* A,
+                 int32_t a_rows, bool dense_root, const Variant& vr, void* ws,
+                 hipStream_t st) {
+  switch (R) {
+    case 16: return launch_typed<N, 16>(B, F, R, A, a_rows, dense_root, vr, ws, st);
+    case 32: return launch_typed<N, 32>(B, F, R, A, a_rows, dense_root, vr, ws, st);
+    case 64: return launch_typed<N, 64>(B, F, R, A, a_rows, dense_root, vr, ws, st);
+    case 128: return launch_typed<N, 128>(B, F, R, A, a_rows, dense_root, vr, ws, st);
+    default: return launch_typed<N, 0>(B, F, R, A, a_rows, dense_root, vr, ws, st);
+  }
+}
+
+}  // namespace
+
+int launch_fill(float* p, int64_t n, float value, hipStream_t st) {
+  if (n <= 0) return TENTACO_OK;
+  fill_kernel<<<grid_for(n / 4 + 1, 256, 16384), 256, 0, st>>>(p, n, value);
+  TT_HIP_CHECK(hipGetLastError());
+  return TENTACO_OK;
+}
+
+Variant variant_of(int v) {
+  constexpr int per_family = kNumGridBlocks * kNumGridGroups;
+  if (v < 0 || v >= kNumVariants) return Variant{-1, 0, 0};
+  if (v < 2 * per_family) {
+    const int fam = v / per_family;
+    const int r = v % per_family;
+    return Variant{fam, kGridBlocks[r / kNumGridGroups], kGridGroups[r % kNumGridGroups]};
+  }
+  return Variant{kSliceRow, kGridBlocks[v - 2 * per_family], 0};
+}
+
+int default_variant() {
+  // eb_sr at block 512, 4 nonzeros per group: the TACO schedule of
+  // mttkrp3_csf_gpu_taco.h:179-207
+  return 0 * (kNumGridBlocks * kNumGridGroups) + 2 * kNumGridGroups + 0;
+}
+
+int auto_variant(const CsfStats& s, int R) {
+  (void)R;
+  constexpr int per_family = kNumGridBlocks * kNumGridGroups;
+  // slice-row pays off when there are enough non-empty slices to fill the
+  // chip and no slice dominates (the longest is within 8x of the mean)
+  if (s.nonempty_slices >= 2048 && s.nnz > 0) {
+    const double mean = double(s.nnz) / s.nonempty_slices;
+    if (double(s.max_slice_nnz) <= 8.0 * mean) return 2 * per_family + 1;  // sr_b256
+  }
+  return 0 * per_family + 1 * kNumGridGroups + 2;  // es_b256_g16
+}
+
+int lanes_per_row(int R) {
+  switch (R) {
+    case 16: return Lay<16>::LPR;
+    case 32: return Lay<32>::LPR;
+    case 64: return Lay<64>::LPR;
+    case 128: return Lay<128>::LPR;
+    default: return Lay<0>::LPR;
+  }
+}
+
+int64_t nnz_per_block(const Variant& vr, int R) {
+  return int64_t(vr.block / lanes_per_row(R)) * vr.group;
+}
+
+size_t mttkrp_workspace_bytes(const CsfDev& B, int R, const Variant& vr) {
+  if (vr.family == kSliceRow || vr.family < 0) return 0;
+  const int64_t NB = nnz_per_block(vr, R);
+  const int64_t nblk = (int64_t(B.nnz) + NB - 1) / NB;
+  return size_t(nblk + 1) * size_t(B.order - 1) * sizeof(int32_t);
+}
+
+int launch_mttkrp(const CsfDev& B, const FactorsDev& F, int R, float* A,
+                  int32_t a_rows, bool dense_root, const Variant& vr, void* ws,
+                  hipStream_t st) {
+  if (R <= 0 || R > Lay<0>::kMaxRank)
+    return fail(TENTACO_EUNSUPPORTED, "rank R=%d outside 1..%d", R, Lay<0>::kMaxRank);
+  if (vr.family < 0) return fail(TENTACO_EINVAL, "invalid variant");
+  switch (B.order) {
+    case 2: return launch_order<2>(B, F, R, A, a_rows, dense_root, vr, ws, st);
+    case 3: return launch_order<3>(B, F, R, A, a_rows, dense_root, vr, ws, st);
+    case 4: return launch_order<4>(B, F, R, A, a_rows, dense_root, vr, ws, st);
+    case 5: return launch_order<5>(B, F, R, A, a_rows, dense_root, vr, ws, st);
+    default: return fail(TENTACO_EUNSUPPORTED, "order %d outside 2..5", B.order);
+  }
+}
+
+}  // namespace tt

@@ -1,0 +1,140 @@
+"""C-ABI checks that need no GPU: the library loads, exports every function
+include/*.h declares, taco_tensor_t matches the reference layout, the
+variant table is consistent, and invalid operands are rejected with codes
+before any device call."""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import tentaco_amd as tt
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in ("tentaco.h", "taco_tensor_t.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        text = text.replace("\\\n", " ")
+        text = re.sub(r"^\s*#[^\n]*", "", text, flags=re.M)
+        for m in re.finditer(r"TENTACO_DECL_VARIANT\((\w+)\)", text):
+            names.add(m.group(1))
+        for m in re.finditer(r"^\s*[A-Za-z_][\w\s\*]*?\b(\w+)\s*\(", text, flags=re.M):
+            name = m.group(1)
+            if name not in ("if", "sizeof", "offsetof", "static_assert", "_Static_assert",
+                            "TENTACO_DECL_VARIANT"):
+                names.add(name)
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(tt.LIB_PATH)
+    names = declared_functions()
+    assert len(names) > 90  # compute API + 66 reference names + handle API
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_reference_variant_names_all_exported():
+    # gpu_kernels.cuh:7-73 declares 67 prototypes (one duplicated)
+    lib = ctypes.CDLL(tt.LIB_PATH)
+    for tb in (256, 512):
+        for g in (4, 8, 16, 32):
+            for c in (1, 2, 4, 8):
+                assert hasattr(lib, f"mttkrp_32_eb_sr_{tb}_{g}_{c}")
+    for nnz in (32, 64, 128, 256):
+        for gs in (4, 8, 16, 32):
+            assert hasattr(lib, f"mttkrp_32_eb_pr_256_{nnz}_{gs}")
+    for nnz in (64, 128, 256, 512):
+        for gs in (4, 8, 16, 32):
+            assert hasattr(lib, f"mttkrp_32_eb_pr_512_{nnz}_{gs}")
+    for n in ("mttkrp_gpu_taco", "ttv_csf_gpu_taco", "get_gpu_timer_result",
+              "mttkrp_32_eb_pr_256_64_32_manual", "mttkrp_64_eb_pr_256_32_32"):
+        assert hasattr(lib, n)
+
+
+def test_taco_tensor_t_layout():
+    # taco_tensor_t.h:6-15: 64 bytes, offsets 0/8/16/24/32/40/48/56
+    T = tt.TacoTensor
+    assert ctypes.sizeof(T) == 64
+    offs = [getattr(T, f).offset for f, _ in T._fields_]
+    assert offs == [0, 8, 16, 24, 32, 40, 48, 56]
+
+
+def test_variant_table():
+    names = tt.variants()
+    assert len(names) == 33
+    assert len(set(names)) == 33
+    seen = set()
+    for v, name in enumerate(names):
+        fam, blk, grp = tt.variant_params(v)
+        assert blk in (128, 256, 512)
+        if fam in (0, 1):
+            assert grp in (4, 8, 16, 32, 64)
+            assert name == f"{'es' if fam == 0 else 'ep'}_b{blk}_g{grp}"
+        else:
+            assert fam == 2 and name == f"sr_b{blk}"
+        seen.add((fam, blk, grp))
+    assert len(seen) == 33
+    # fixed default = the TACO schedule eb_sr 512 / g=4
+    assert names[tt.default_variant()] == "es_b512_g4"
+
+
+def _bad_csf_cases():
+    base = tt.Csf(3, (4, 4, 4),
+                  [np.array([0, 4], np.int32), np.arange(4, dtype=np.int32),
+                   np.array([0, 1, 2], np.int32)],
+                  [None, None, None], None)
+    rng = np.random.default_rng(0)
+    coords = np.array([[0, 1, 3], [1, 2, 0], [3, 0, 2]], np.int32)
+    good = tt.Csf.from_coo((4, 4, 4), coords, rng.uniform(-1, 1, 3).astype(np.float32))
+    cases = []
+    # decreasing pos
+    b = tt.Csf(3, good.dims, [p.copy() for p in good.pos], [c.copy() for c in good.crd], good.vals.copy())
+    b.pos[1][2] = b.pos[1][1] - 1 if b.pos[1][1] > 0 else 5
+    cases.append(("pos decreases", b))
+    # coordinate out of range
+    b = tt.Csf(3, good.dims, [p.copy() for p in good.pos], [c.copy() for c in good.crd], good.vals.copy())
+    b.crd[2][0] = 4
+    cases.append(("crd out of range", b))
+    # root crd not increasing
+    b = tt.Csf(3, good.dims, [p.copy() for p in good.pos], [c.copy() for c in good.crd], good.vals.copy())
+    b.crd[0] = np.array([0, 2, 1, 3], np.int32)
+    cases.append(("root order", b))
+    # root pos malformed
+    b = tt.Csf(3, good.dims, [p.copy() for p in good.pos], [c.copy() for c in good.crd], good.vals.copy())
+    b.pos[0] = np.array([1, 4], np.int32)
+    cases.append(("root pos", b))
+    del base
+    return cases
+
+
+@pytest.mark.parametrize("label,B", _bad_csf_cases())
+def test_invalid_csf_rejected_before_any_device_call(label, B):
+    # validation runs on the host before hipMalloc, so this needs no GPU
+    with pytest.raises(tt.TentacoError) as ei:
+        tt.DeviceCsf(B)
+    assert ei.value.code == tt.EINVAL, (label, str(ei.value))
+
+
+def test_unsupported_order_rejected():
+    rng = np.random.default_rng(1)
+    coords = rng.integers(0, 3, (3, 5)).astype(np.int32)
+    B = tt.Csf.from_coo((3, 3, 3), coords, np.ones(5, np.float32))
+    t, keep = B.as_taco()
+    h = ctypes.c_void_p()
+    rc = tt.lib().tentaco_csf_upload(ctypes.byref(t), 6, ctypes.byref(h))
+    assert rc == tt.EUNSUPPORTED
+    assert b"order" in tt.lib().tentaco_last_error()
+
+
+def test_builder_rejects_out_of_range_coords():
+    coords = np.array([[0, 5], [0, 1]], np.int32)
+    with pytest.raises(tt.TentacoError) as ei:
+        tt.Csf.from_coo((4, 4), coords, np.ones(2, np.float32))
+    assert ei.value.code == tt.EINVAL

@@ -1,0 +1,248 @@
+"""GPU parity of the MTTKRP kernels against the CPU oracle (restated
+mttkrp{3,4,5}_csf_cpu_taco), through the C ABI.
+
+Tolerance: FP32 results vs the FP32 restatement and vs the FP64 (reference
+typing) restatement, relative L2 <= 1e-5 (REL_L2_TOL). Integer-valued known
+answers (all-ones) must be bit-exact.
+"""
+
+import numpy as np
+import pytest
+
+import oracle
+import tentaco_amd as tt
+from helpers import REL_L2_TOL, factors_for, random_coo, random_csf, skewed_coo
+
+pytestmark = pytest.mark.gpu
+
+ALL_VARIANTS = list(range(33))
+
+
+def _run_dev(B, facs, variant, R, a_rows=None):
+    H = tt.DeviceCsf(B)
+    fd = [tt.DeviceArray.from_numpy(f) for f in facs]
+    rows = B.dims[0] if a_rows is None else a_rows
+    A = tt.DeviceArray.empty((rows, R))
+    H.mttkrp(fd, A, R, variant, a_rows=rows)
+    out = A.numpy()
+    H.close()
+    return out
+
+
+def _check(B, facs, got, label):
+    ref32 = oracle.mttkrp(B, facs, "taco", fp64=False, out_rows=got.shape[0])
+    ref64 = oracle.mttkrp(B, facs, "taco", fp64=True, out_rows=got.shape[0])
+    e32 = oracle.rel_l2(got, ref32)
+    e64 = oracle.rel_l2(got, ref64)
+    assert e32 <= REL_L2_TOL and e64 <= REL_L2_TOL, f"{label}: rel_l2 fp32={e32:.2e} fp64={e64:.2e}"
+    return e64
+
+
+@pytest.mark.parametrize("order", [2, 3, 4, 5])
+@pytest.mark.parametrize("R", [16, 32, 64, 7])
+def test_all_variants_random(gpu, order, R):
+    rng = np.random.default_rng(100 * order + R)
+    dims = {2: (300, 200), 3: (120, 90, 80), 4: (40, 30, 25, 20), 5: (20, 15, 12, 10, 9)}[order]
+    B = random_csf(rng, dims, 6000)
+    facs = factors_for(rng, B, R)
+    H = tt.DeviceCsf(B)
+    fd = [tt.DeviceArray.from_numpy(f) for f in facs]
+    A = tt.DeviceArray.empty((B.dims[0], R))
+    ref32 = oracle.mttkrp(B, facs, "taco", fp64=False)
+    ref64 = oracle.mttkrp(B, facs, "taco", fp64=True)
+    for v in ALL_VARIANTS:
+        H.mttkrp(fd, A, R, v)
+        got = A.numpy()
+        e32, e64 = oracle.rel_l2(got, ref32), oracle.rel_l2(got, ref64)
+        assert e32 <= REL_L2_TOL and e64 <= REL_L2_TOL, (tt.variants()[v], e32, e64)
+
+
+@pytest.mark.parametrize("order", [3, 4, 5])
+def test_skewed_giant_slice_and_empty_slices(gpu, order):
+    """One slice holds 60% of the nonzeros (runs and workgroups inside one
+    slice) and a third of the slices are empty (empty nodes inside the
+    partition brackets)."""
+    rng = np.random.default_rng(7 + order)
+    dims = {3: (97, 300, 400), 4: (61, 40, 50, 60), 5: (31, 20, 20, 20, 20)}[order]
+    coords, vals = skewed_coo(rng, dims, 40000, hot_slice=5)
+    B = tt.Csf.from_coo(dims, coords, vals)
+    assert np.count_nonzero(np.diff(B.pos[1]) == 0) > dims[0] // 4
+    for R in (32, 64):
+        facs = factors_for(rng, B, R)
+        H = tt.DeviceCsf(B)
+        fd = [tt.DeviceArray.from_numpy(f) for f in facs]
+        A = tt.DeviceArray.empty((B.dims[0], R))
+        ref64 = oracle.mttkrp(B, facs, "taco", fp64=True)
+        for v in ALL_VARIANTS:
+            H.mttkrp(fd, A, R, v)
+            e = oracle.rel_l2(A.numpy(), ref64)
+            assert e <= REL_L2_TOL, (tt.variants()[v], R, e)
+
+
+def test_long_fibers_span_workgroups(gpu):
+    """3-mode tensor whose fibers hold thousands of nonzeros, so single fibers
+    span many runs and workgroups (partial fiber sums times C rows)."""
+    rng = np.random.default_rng(3)
+    dims = (8, 4, 50000)
+    n = 60000
+    coords = np.stack([rng.integers(0, 8, n), rng.integers(0, 4, n), rng.integers(0, 50000, n)]).astype(np.int32)
+    vals = rng.uniform(-1, 1, n).astype(np.float32)
+    B = tt.Csf.from_coo(dims, coords, vals)
+    facs = factors_for(rng, B, 32)
+    for v in ALL_VARIANTS:
+        _check(B, facs, _run_dev(B, facs, v, 32), tt.variants()[v])
+
+
+@pytest.mark.parametrize("order", [3, 4, 5])
+def test_all_ones_kat_exact(gpu, order):
+    # mttkrp_csf_gpu.h:42: B = C = D = 1 -> A[i, :] = nnz(slice i), exact
+    rng = np.random.default_rng(40 + order)
+    dims = {3: (64, 50, 50), 4: (32, 20, 20, 20), 5: (16, 10, 10, 10, 10)}[order]
+    coords, _ = random_coo(rng, dims, 20000)
+    B = tt.Csf.from_coo(dims, coords, np.ones(coords.shape[1], np.float32))
+    B = tt.Csf(B.order, B.dims, B.pos, B.crd, np.ones_like(B.vals), _owner=B._owner)
+    counts = np.diff(oracle.coo_from_csf(B)[0][0].searchsorted(np.arange(dims[0] + 1)))
+    for R in (16, 32):
+        facs = [np.ones((dims[m], R), np.float32) for m in range(1, order)]
+        for v in ALL_VARIANTS:
+            got = _run_dev(B, facs, v, R)
+            assert np.array_equal(got, np.repeat(counts[:, None].astype(np.float32), R, 1)), \
+                tt.variants()[v]
+
+
+def test_one_hot_factor_selects_slices(gpu):
+    rng = np.random.default_rng(11)
+    dims = (50, 40, 30)
+    B = random_csf(rng, dims, 5000)
+    T = oracle.dense_from_csf(B)
+    C = np.zeros((40, 16), np.float32)
+    D = np.zeros((30, 16), np.float32)
+    for r in range(16):
+        C[(3 * r) % 40, r] = 1
+        D[(7 * r) % 30, r] = 1
+    for v in (tt.default_variant(), 5, 20, 31):
+        got = _run_dev(B, [C, D], v, 16)
+        exp = np.stack([T[:, (3 * r) % 40, (7 * r) % 30] for r in range(16)], 1)
+        assert np.array_equal(got, exp.astype(np.float32)), tt.variants()[v]
+
+
+def test_empty_and_tiny_tensors(gpu):
+    rng = np.random.default_rng(5)
+    # nnz = 0: A must come back all zeros (overwritten, not left uninitialised)
+    B = tt.Csf.from_coo((10, 5, 5), np.zeros((3, 0), np.int32), np.zeros(0, np.float32))
+    facs = factors_for(rng, B, 32)
+    for v in ALL_VARIANTS:
+        H = tt.DeviceCsf(B)
+        fd = [tt.DeviceArray.from_numpy(f) for f in facs]
+        A = tt.DeviceArray.from_numpy(np.full((10, 32), 7.0, np.float32))
+        H.mttkrp(fd, A, 32, v)
+        assert np.array_equal(A.numpy(), np.zeros((10, 32), np.float32)), tt.variants()[v]
+    # a single nonzero
+    B = tt.Csf.from_coo((10, 5, 5), np.array([[3], [2], [4]], np.int32), np.array([0.5], np.float32))
+    facs = factors_for(rng, B, 32)
+    for v in ALL_VARIANTS:
+        _check(B, facs, _run_dev(B, facs, v, 32), tt.variants()[v])
+
+
+def test_compressed_root(gpu):
+    """Root stored compressed (n0 < dims[0], crd0 sparse): rows not listed in
+    crd0 must be zero, listed rows land at crd0[x]."""
+    rng = np.random.default_rng(9)
+    dims = (1000, 60, 70)
+    coords, vals = random_coo(rng, (40, 60, 70), 8000)
+    rows = np.sort(rng.choice(1000, 40, replace=False)).astype(np.int32)
+    coords[0] = rows[coords[0]]
+    D = tt.Csf.from_coo(dims, coords, vals)
+    nonempty = np.flatnonzero(np.diff(D.pos[1])).astype(np.int32)
+    pos1 = np.append(D.pos[1][nonempty], D.pos[1][-1]).astype(np.int32)
+    Bc = tt.Csf(3, dims, [np.array([0, nonempty.size], np.int32), pos1, D.pos[2]],
+                [nonempty, D.crd[1], D.crd[2]], D.vals)
+    facs = factors_for(rng, D, 32)
+    ref = oracle.mttkrp(D, facs, "taco", fp64=True)
+    for v in ALL_VARIANTS:
+        H = tt.DeviceCsf(Bc)
+        fd = [tt.DeviceArray.from_numpy(f) for f in facs]
+        A = tt.DeviceArray.from_numpy(np.full((1000, 32), 3.0, np.float32))
+        H.mttkrp(fd, A, 32, v)
+        e = oracle.rel_l2(A.numpy(), ref)
+        assert e <= REL_L2_TOL, (tt.variants()[v], e)
+
+
+def test_host_entry_points_and_reference_names(gpu):
+    """tentaco_mttkrp3/4/5 and the reference's symbol names (gpu_kernels.cuh)
+    fill A in place and set get_gpu_timer_result."""
+    import ctypes
+
+    rng = np.random.default_rng(21)
+    B = random_csf(rng, (80, 60, 50), 7000)
+    facs = factors_for(rng, B, 32)
+    ref = oracle.mttkrp(B, facs, "taco", fp64=True)
+    got = tt.mttkrp(B, facs)
+    assert oracle.rel_l2(got, ref) <= REL_L2_TOL
+    assert tt.last_timer_ms() > 0
+    L = tt.lib()
+    tB, kB = B.as_taco()
+    tC, kC = tt.dense_taco(facs[0])
+    tD, kD = tt.dense_taco(facs[1])
+    for name in ("mttkrp_gpu_taco", "mttkrp_32_eb_sr_512_4_1", "mttkrp_32_eb_sr_256_32_8",
+                 "mttkrp_32_eb_pr_256_64_32_manual", "mttkrp_32_eb_pr_512_512_32",
+                 "mttkrp_64_eb_pr_256_32_32"):
+        A = np.full((80, 32), -1.0, np.float32)
+        tA, kA = tt.dense_taco(A)
+        fn = getattr(L, name)
+        fn.restype = None
+        fn(ctypes.byref(tA), ctypes.byref(tB), ctypes.byref(tC), ctypes.byref(tD))
+        assert oracle.rel_l2(A, ref) <= REL_L2_TOL, name
+        assert L.get_gpu_timer_result() > 0
+    # tentaco_mttkrp3 / 4 / 5
+    tA_arr = np.zeros((80, 32), np.float32)
+    tA, kA = tt.dense_taco(tA_arr)
+    assert L.tentaco_mttkrp3(ctypes.byref(tA), ctypes.byref(tB), ctypes.byref(tC), ctypes.byref(tD)) == 0
+    assert oracle.rel_l2(tA_arr, ref) <= REL_L2_TOL
+    for order in (4, 5):
+        dims = (30, 12, 11, 10, 9)[:order]
+        B4 = random_csf(rng, dims, 3000)
+        f4 = factors_for(rng, B4, 16)
+        r4 = oracle.mttkrp(B4, f4, "taco", fp64=True)
+        out = np.zeros((30, 16), np.float32)
+        tO, kO = tt.dense_taco(out)
+        tB4, kB4 = B4.as_taco()
+        tf = [tt.dense_taco(f) for f in f4]
+        fn = L.tentaco_mttkrp4 if order == 4 else L.tentaco_mttkrp5
+        assert fn(ctypes.byref(tO), ctypes.byref(tB4), *[ctypes.byref(t) for t, _ in tf]) == 0
+        assert oracle.rel_l2(out, r4) <= REL_L2_TOL
+
+
+def test_host_entry_rejects_bad_factor(gpu):
+    rng = np.random.default_rng(22)
+    B = random_csf(rng, (20, 30, 40), 500)
+    facs = factors_for(rng, B, 32)
+    short = facs[1][:10]  # fewer rows than the coordinates reach
+    with pytest.raises(tt.TentacoError) as ei:
+        tt.mttkrp(B, [facs[0], short])
+    assert ei.value.code == tt.EINVAL
+
+
+def test_slice_row_is_deterministic(gpu):
+    rng = np.random.default_rng(31)
+    B = random_csf(rng, (500, 300, 300), 50000)
+    facs = factors_for(rng, B, 32)
+    sr = [v for v in ALL_VARIANTS if tt.variant_params(v)[0] == 2]
+    outs = [_run_dev(B, facs, sr[0], 32) for _ in range(3)]
+    assert all(np.array_equal(outs[0], o) for o in outs[1:])
+
+
+def test_repeated_calls_reuse_handle(gpu):
+    """CP-ALS-style reuse: one upload, many calls with changing factors and
+    variants; every call fully overwrites A."""
+    rng = np.random.default_rng(41)
+    B = random_csf(rng, (200, 100, 100), 30000)
+    H = tt.DeviceCsf(B)
+    A = tt.DeviceArray.empty((200, 32))
+    for it in range(6):
+        facs = factors_for(rng, B, 32)
+        fd = [tt.DeviceArray.from_numpy(f) for f in facs]
+        v = [tt.VARIANT_AUTO, 3, 17, 31, tt.VARIANT_DEFAULT, 12][it]
+        H.mttkrp(fd, A, 32, v)
+        e = oracle.rel_l2(A.numpy(), oracle.mttkrp(B, facs, "taco", fp64=True))
+        assert e <= REL_L2_TOL, (it, v, e)

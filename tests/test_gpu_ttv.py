@@ -1,0 +1,101 @@
+"""GPU parity of TTV (ttv_csf_gpu_taco replacement) against the oracle
+(restated ttv_csf_cpu_taco) and closed-form known answers."""
+
+import numpy as np
+import pytest
+
+import oracle
+import tentaco_amd as tt
+from helpers import REL_L2_TOL, random_coo, random_csf, skewed_coo
+
+pytestmark = pytest.mark.gpu
+
+
+def _ttv_dev(B, c):
+    H = tt.DeviceCsf(B)
+    cd = tt.DeviceArray.from_numpy(c.astype(np.float32))
+    y = tt.DeviceArray.from_numpy(np.full(B.nodes(B.order - 2), 5.0, np.float32))
+    H.ttv(cd, y)
+    return y.numpy()
+
+
+@pytest.mark.parametrize("order", [2, 3, 4, 5])
+def test_ttv_random(gpu, order):
+    rng = np.random.default_rng(60 + order)
+    dims = {2: (400, 3000), 3: (100, 80, 2000), 4: (30, 20, 20, 500), 5: (12, 10, 10, 10, 300)}[order]
+    B = random_csf(rng, dims, 30000)
+    c = rng.uniform(-1, 1, dims[-1]).astype(np.float32)
+    got = _ttv_dev(B, c)
+    for fp64 in (False, True):
+        ref = oracle.ttv(B, c, "taco", fp64=fp64)
+        assert oracle.rel_l2(got, ref) <= REL_L2_TOL
+    # host entry point (tentaco_ttv_n) gives the same result
+    got_h = tt.ttv(B, c)
+    assert oracle.rel_l2(got_h, oracle.ttv(B, c, "taco", True)) <= REL_L2_TOL
+
+
+def test_ttv_long_and_short_fibers(gpu):
+    """Fibers from 1 to ~20k nonzeros: fibers inside one lane, spanning lanes,
+    wavefronts and workgroups."""
+    rng = np.random.default_rng(71)
+    parts = []
+    for (nslices, nfib, L) in ((5, 2, 20000), (50, 30, 3), (20, 100, 1), (10, 10, 700)):
+        parts.append((nslices, nfib, L))
+    coords = []
+    base = 0
+    for nslices, nfib, L in parts:
+        for s in range(nslices):
+            for f in range(nfib):
+                ks = rng.choice(30000, size=L, replace=False)
+                coords.append(np.stack([np.full(L, base + s), np.full(L, f), ks]))
+        base += nslices
+    coords = np.concatenate(coords, 1).astype(np.int32)
+    vals = rng.uniform(-1, 1, coords.shape[1]).astype(np.float32)
+    B = tt.Csf.from_coo((base, 100, 30000), coords, vals)
+    c = rng.uniform(-1, 1, 30000).astype(np.float32)
+    got = _ttv_dev(B, c)
+    assert oracle.rel_l2(got, oracle.ttv(B, c, "taco", True)) <= REL_L2_TOL
+    # all ones -> fiber lengths exactly
+    Bo = tt.Csf(B.order, B.dims, B.pos, B.crd, np.ones_like(B.vals), _owner=B._owner)
+    ones = _ttv_dev(Bo, np.ones(30000, np.float32))
+    assert np.array_equal(ones, np.diff(B.pos[2]).astype(np.float32))
+
+
+def test_ttv_one_hot(gpu):
+    rng = np.random.default_rng(72)
+    B = random_csf(rng, (60, 50, 40), 5000)
+    k = 17
+    e = np.zeros(40, np.float32)
+    e[k] = 1
+    got = _ttv_dev(B, e)
+    exp = oracle.ttv(B, e, "taco", True)
+    assert np.array_equal(got, exp.astype(np.float32))
+
+
+def test_ttv_empty_and_skewed(gpu):
+    rng = np.random.default_rng(73)
+    B = tt.Csf.from_coo((10, 5, 5), np.zeros((3, 0), np.int32), np.zeros(0, np.float32))
+    H = tt.DeviceCsf(B)
+    cd = tt.DeviceArray.from_numpy(np.ones(5, np.float32))
+    y = tt.DeviceArray.empty((1,))
+    H.ttv(cd, y)  # no fibers: nothing to write, must not fault
+    coords, vals = skewed_coo(rng, (50, 200, 3000), 60000)
+    B = tt.Csf.from_coo((50, 200, 3000), coords, vals)
+    c = rng.uniform(-1, 1, 3000).astype(np.float32)
+    got = _ttv_dev(B, c)
+    assert oracle.rel_l2(got, oracle.ttv(B, c, "taco", True)) <= REL_L2_TOL
+
+
+def test_ttv_reference_name(gpu):
+    import ctypes
+
+    rng = np.random.default_rng(74)
+    B = random_csf(rng, (40, 30, 20), 3000)
+    c = rng.uniform(-1, 1, 20).astype(np.float32)
+    y = np.zeros(B.nodes(1), np.float32)
+    tA, kA = tt.dense_taco(y)
+    tB, kB = B.as_taco()
+    tc, kc = tt.dense_taco(c)
+    fn = tt.lib().ttv_csf_gpu_taco
+    fn(ctypes.byref(tA), ctypes.byref(tB), ctypes.byref(tc))
+    assert oracle.rel_l2(y, oracle.ttv(B, c, "taco", True)) <= REL_L2_TOL
