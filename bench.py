@@ -103,6 +103,98 @@ def run_mttkrp_workload(tt, B, R, seed, K, W, reps=5, check=True, fp32_check=Tru
     return res, facs
 
 
+def run_batch_workload(tt, metrics, oracle, K):
+    """Every mode of 64 uneven tensors (R=32): one persistent launch in a
+    hipGraph vs one launch per tensor-mode (each with its own autotuned
+    variant)."""
+    import numpy as np
+
+    log("generating the 64-tensor uneven batch")
+    t0 = time.time()
+    tensors = metrics.workload_batch()
+    R = 32
+    rng = np.random.default_rng(1004)
+    tasks, hosts, facs_d, outs, rows = [], [], [], [], []
+    keep = []
+    for B in tensors:
+        Fd = [tt.DeviceArray.from_numpy(rng.uniform(-1, 1, (d, R)).astype(np.float32)) for d in B.dims]
+        keep.append(Fd)
+        for m, Bm in enumerate(B.mode_csfs()):
+            rest = [k for k in range(B.order) if k != m]
+            tasks.append(tt.DeviceCsf(Bm))
+            hosts.append((Bm, m, rest))
+            facs_d.append([Fd[k] for k in rest])
+            outs.append(tt.DeviceArray.empty((B.dims[m], R)))
+            rows.append(B.dims[m])
+    visits = sum(h[0].nnz for h in hosts)
+    balg = sum(metrics.alg_bytes_mttkrp(h[0], R) for h in hosts)
+    log(f"{len(tasks)} tasks, {visits} nonzero-mode visits, built in {time.time()-t0:.1f}s")
+    # per-task baseline: each task autotuned, then one launch sequence per task
+    for H, fd, o in zip(tasks, facs_d, outs):
+        H.autotune(fd, o, R, reps=2)
+        H.reserve(R, tt.VARIANT_AUTO)
+
+    def per_task():
+        for H, fd, o in zip(tasks, facs_d, outs):
+            H.mttkrp(fd, o, R, tt.VARIANT_AUTO)
+
+    per_task()
+    ref = [o.numpy() for o in outs]
+    per_ms, _ = time_steps(per_task, max(3, K // 5))
+    per_ms /= max(3, K // 5)
+    bt = tt.Batch(tasks, R)
+    bt.set_operands(facs_d, outs, rows)
+    (fam, blk, grp), tune_ms = bt.autotune(reps=3)
+    items, grid, nb = bt.items()
+    bt.run()
+    tt.synchronize()
+    eager_err = max(oracle.rel_l2(o.numpy(), r) for o, r in zip(outs, ref))
+
+    def eager():
+        bt.run()
+
+    eager_ms, _ = time_steps(eager, max(3, K // 5))
+    eager_ms /= max(3, K // 5)
+    bt.capture()
+
+    def graph():
+        bt.launch()
+
+    for _ in range(2):
+        graph()
+    g_ms, _ = time_steps(graph, K)
+    g_ms /= K
+    graph_err = max(oracle.rel_l2(o.numpy(), r) for o, r in zip(outs, ref))
+    # oracle spot check on the three smallest tasks
+    small = sorted(range(len(hosts)), key=lambda i: hosts[i][0].nnz)[:3]
+    spot = []
+    for i in small:
+        Bm, m, rest = hosts[i]
+        fh = [f.numpy() for f in facs_d[i]]
+        spot.append(oracle.rel_l2(outs[i].numpy(), oracle.mttkrp(Bm, fh, "taco", fp64=True)))
+    res = {
+        "tensors": len(tensors),
+        "tasks": len(tasks),
+        "nnz_mode_visits": visits,
+        "alg_bytes": balg,
+        "per_task_launch_ms": round(per_ms, 4),
+        "batched_eager_ms": round(eager_ms, 4),
+        "batched_graph_ms": round(g_ms, 4),
+        "speedup_graph_vs_per_task": round(per_ms / g_ms, 3),
+        "batched_gbps": round(balg / (g_ms * 1e-3) / 1e9, 2),
+        "config": {"family": ["edge-serial", "edge-parallel"][fam], "block": blk, "group": grp,
+                   "items": items, "persistent_grid": grid, "nnz_per_item": nb},
+        "max_rel_l2_batched_vs_per_task": max(eager_err, graph_err),
+        "oracle_rel_l2_smallest_tasks": spot,
+    }
+    log(f"batch: per-task {per_ms:.3f} ms, batched graph {g_ms:.3f} ms "
+        f"({res['speedup_graph_vs_per_task']}x)")
+    bt.close()
+    for H in tasks:
+        H.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -211,6 +303,9 @@ def main():
         }
         log(f"TTV 20M: {extra['c2_ttv']['gbps']} GB/s, rel_l2 {e2:.2e}")
         H2.close()
+
+    if not args.quick:
+        extra["c4_batch"] = run_batch_workload(tt, metrics, oracle, K)
 
     line = {
         "metric": "MTTKRP-3 R=32 GB/s (% HBM peak); learned-pick time / sweep-best time",

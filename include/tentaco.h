@@ -222,6 +222,44 @@ int tentaco_csf_set_tuned(tentaco_csf* h, int R, int variant);
 int tentaco_ttv_dev(tentaco_csf* h, const float* c_dev, float* A_dev,
                     void* stream);
 
+/* ---- batched persistent launch ----------------------------------------- */
+/* Many MTTKRP tasks (e.g. every mode of every tensor of a CP-ALS sweep, one
+ * CSF per mode ordering) in one persistent kernel: all tasks' nonzeros are cut
+ * into equal work items that persistent workgroups pull from a device
+ * counter, so uneven tensor sizes do not leave the GPU idle and the whole
+ * batch costs two launches (prep + persistent), or one graph launch. No
+ * counterpart in the reference, which launches per tensor-mode
+ * (mttkrp_csf_gpu.h:186-217). */
+typedef struct tentaco_batch tentaco_batch;
+/* tasks: ntasks device-resident CSFs (kept by pointer, must outlive the
+ * batch); R in {16, 32, 64}. */
+int tentaco_batch_create(tentaco_csf* const* tasks, int ntasks, int R,
+                         tentaco_batch** out);
+/* factors_dev: ntasks x 4 device pointers, task-major; slot m-1 of task t is
+ * the factor of level m (slots past order-1 ignored). outs_dev[t] holds
+ * out_rows[t] x R floats and is fully overwritten by each run. */
+int tentaco_batch_set_operands(tentaco_batch* b, const float* const* factors_dev,
+                               float* const* outs_dev, const int32_t* out_rows);
+/* Tile configuration: family 0 edge-serial / 1 edge-parallel, block in
+ * {128, 256, 512}, group = nonzeros per lane group. Drops a captured graph. */
+int tentaco_batch_configure(tentaco_batch* b, int family, int block, int group);
+/* Enqueue prep + persistent kernel on `stream`. */
+int tentaco_batch_run(tentaco_batch* b, void* stream);
+/* Capture tentaco_batch_run into an instantiated hipGraph (kernels only). */
+int tentaco_batch_capture(tentaco_batch* b, void* stream);
+/* Launch the captured graph on `stream`. */
+int tentaco_batch_launch(tentaco_batch* b, void* stream);
+/* Work items, persistent grid size and nonzeros per item of the current
+ * configuration. */
+int tentaco_batch_items(const tentaco_batch* b, int32_t* items, int32_t* grid,
+                        int32_t* nb);
+/* Time every tile configuration (2 families x 3 blocks x 5 groups), keep the
+ * fastest configured and return it. Not for graph capture. */
+int tentaco_batch_autotune(tentaco_batch* b, int reps, void* stream,
+                           int* best_family, int* best_block, int* best_group,
+                           float* best_ms);
+int tentaco_batch_free(tentaco_batch* b);
+
 /* Device memory helpers for hosts without their own HIP runtime binding. */
 int tentaco_dev_malloc(void** ptr, size_t bytes);
 int tentaco_dev_free(void* ptr);
@@ -270,6 +308,13 @@ typedef struct tentaco_synth_params {
 } tentaco_synth_params;
 int tentaco_synth_csf(const tentaco_synth_params* p, tentaco_host_csf** out);
 
+/* CSF of the same tensor with its modes in another order: level m of the
+ * result is mode mode_order[m] of B (so mode_order[0] becomes the root and
+ * the MTTKRP of that mode). Values travel with their coordinates; the
+ * result is lexicographic with a dense root. B needs a valid dimensions
+ * array and CSF levels (order given explicitly). */
+int tentaco_csf_reorder(const taco_tensor_t* B, int order, const int* mode_order,
+                        tentaco_host_csf** out);
 /* View valid until tentaco_host_csf_free; order field is set. */
 taco_tensor_t* tentaco_host_csf_view(tentaco_host_csf* h);
 /* Number of nodes at a level (level order-1 = nonzeros), -1 if invalid. */

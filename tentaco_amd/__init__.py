@@ -107,6 +107,15 @@ def lib() -> ctypes.CDLL:
         "tentaco_csf_autotune": (c_int, [c_void_p, c_int, POINTER(c_void_p), c_void_p, c_int32, c_int,
                                          c_void_p, POINTER(c_int), POINTER(c_float)]),
         "tentaco_csf_set_tuned": (c_int, [c_void_p, c_int, c_int]),
+        "tentaco_batch_create": (c_int, [POINTER(c_void_p), c_int, c_int, POINTER(c_void_p)]),
+        "tentaco_batch_set_operands": (c_int, [c_void_p, POINTER(c_void_p), POINTER(c_void_p), POINTER(c_int32)]),
+        "tentaco_batch_configure": (c_int, [c_void_p, c_int, c_int, c_int]),
+        "tentaco_batch_run": (c_int, [c_void_p, c_void_p]),
+        "tentaco_batch_capture": (c_int, [c_void_p, c_void_p]),
+        "tentaco_batch_launch": (c_int, [c_void_p, c_void_p]),
+        "tentaco_batch_items": (c_int, [c_void_p, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32)]),
+        "tentaco_batch_autotune": (c_int, [c_void_p, c_int, c_void_p, POINTER(c_int), POINTER(c_int), POINTER(c_int), POINTER(c_float)]),
+        "tentaco_batch_free": (c_int, [c_void_p]),
         "tentaco_dev_malloc": (c_int, [POINTER(c_void_p), c_size_t]),
         "tentaco_dev_free": (c_int, [c_void_p]),
         "tentaco_memcpy_h2d": (c_int, [c_void_p, c_void_p, c_size_t]),
@@ -118,6 +127,7 @@ def lib() -> ctypes.CDLL:
         "tentaco_coo_to_csf": (c_int, [c_int, POINTER(c_int32), c_int64, POINTER(c_int32), POINTER(c_float), POINTER(c_void_p)]),
         "tentaco_synth_csf": (c_int, [POINTER(SynthParams), POINTER(c_void_p)]),
         "tentaco_host_csf_view": (P, [c_void_p]),
+        "tentaco_csf_reorder": (c_int, [P, c_int, POINTER(c_int), POINTER(c_void_p)]),
         "tentaco_host_csf_nodes": (c_int64, [c_void_p, c_int]),
         "tentaco_host_csf_free": (c_int, [c_void_p]),
     }
@@ -206,6 +216,23 @@ class Csf:
             )
         )
         return _wrap_host_csf(h)
+
+    def reorder(self, mode_order: Sequence[int]) -> "Csf":
+        """Same tensor with level m = mode mode_order[m] (tentaco_csf_reorder)."""
+        t, keep = self.as_taco()
+        mo = (c_int * self.order)(*[int(m) for m in mode_order])
+        h = c_void_p()
+        _check(lib().tentaco_csf_reorder(ctypes.byref(t), self.order, mo, ctypes.byref(h)))
+        del keep
+        return _wrap_host_csf(h)
+
+    def mode_csfs(self) -> List["Csf"]:
+        """One CSF per mode ordering: mode m at the root, the others after it
+        in their original order."""
+        out = []
+        for m in range(self.order):
+            out.append(self if m == 0 else self.reorder([m] + [k for k in range(self.order) if k != m]))
+        return out
 
     @classmethod
     def synth(cls, order: int, dims: Sequence[int], nnz: int, slice_s: float = 1.1,
@@ -350,6 +377,13 @@ class DeviceArray:
         n = int(np.prod(shape)) * np.dtype(dtype).itemsize
         return cls(n, dtype, shape)
 
+    def upload(self, a: np.ndarray):
+        """Copy a host array of the same byte size into this buffer."""
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        if a.nbytes:
+            _check(lib().tentaco_memcpy_h2d(self.ptr, a.ctypes.data_as(c_void_p), a.nbytes))
+
     def numpy(self) -> np.ndarray:
         out = np.empty(self.shape, dtype=self.dtype)
         if self.nbytes:
@@ -412,6 +446,66 @@ class DeviceCsf:
     def close(self):
         if self.h:
             lib().tentaco_csf_free(self.h)
+            self.h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Batched persistent MTTKRP over many device-resident CSFs
+    (tentaco_batch_*): one persistent launch for all tasks, optionally
+    captured in a hipGraph."""
+
+    def __init__(self, tasks: Sequence[DeviceCsf], R: int):
+        self.tasks = list(tasks)
+        self.R = R
+        arr = (c_void_p * len(tasks))(*[t.h for t in tasks])
+        self.h = c_void_p()
+        _check(lib().tentaco_batch_create(arr, len(tasks), R, ctypes.byref(self.h)))
+        self._keep = []
+
+    def set_operands(self, factors: Sequence[Sequence[DeviceArray]], outs: Sequence[DeviceArray],
+                     out_rows: Sequence[int]):
+        n = len(self.tasks)
+        fa = (c_void_p * (4 * n))()
+        for t in range(n):
+            for j, f in enumerate(factors[t]):
+                fa[4 * t + j] = f.ptr
+        oa = (c_void_p * n)(*[o.ptr for o in outs])
+        rows = (c_int32 * n)(*[int(r) for r in out_rows])
+        self._keep = [factors, outs]
+        _check(lib().tentaco_batch_set_operands(self.h, fa, oa, rows))
+
+    def configure(self, family: int, block: int, group: int):
+        _check(lib().tentaco_batch_configure(self.h, family, block, group))
+
+    def run(self, stream=None):
+        _check(lib().tentaco_batch_run(self.h, stream))
+
+    def capture(self, stream=None):
+        _check(lib().tentaco_batch_capture(self.h, stream))
+
+    def launch(self, stream=None):
+        _check(lib().tentaco_batch_launch(self.h, stream))
+
+    def items(self):
+        i, g, nb = c_int32(), c_int32(), c_int32()
+        _check(lib().tentaco_batch_items(self.h, ctypes.byref(i), ctypes.byref(g), ctypes.byref(nb)))
+        return i.value, g.value, nb.value
+
+    def autotune(self, reps: int = 3, stream=None):
+        f, b, g, ms = c_int(), c_int(), c_int(), c_float()
+        _check(lib().tentaco_batch_autotune(self.h, reps, stream, ctypes.byref(f), ctypes.byref(b),
+                                            ctypes.byref(g), ctypes.byref(ms)))
+        return (f.value, b.value, g.value), float(ms.value)
+
+    def close(self):
+        if self.h:
+            lib().tentaco_batch_free(self.h)
             self.h = c_void_p()
 
     def __del__(self):

@@ -1,0 +1,300 @@
+// Batched MTTKRP over many (tensor, mode) tasks in one persistent launch, its
+// hipGraph capture and its tile autotuner. The per-task alternative is one
+// tentaco_mttkrp_dev call per task.
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "handle.h"
+#include "internal.h"
+
+struct tentaco_batch {
+  int ntasks = 0;
+  int R = 0;
+  std::vector<tentaco_csf*> tasks;
+  std::vector<tt::BatchTaskDev> host_tasks;
+  bool have_operands = false;
+  // device tables
+  tt::BatchTaskDev* d_tasks = nullptr;
+  int64_t* d_fill_off = nullptr;
+  int64_t* d_bnd_off = nullptr;
+  int64_t* d_item_off = nullptr;
+  int32_t* d_counter = nullptr;
+  int32_t* d_brk = nullptr;
+  size_t brk_bytes = 0;
+  tt::BatchLaunch L;
+  // graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace tt {
+namespace {
+
+template <class T>
+int dev_alloc(T** p, size_t n) {
+  *p = nullptr;
+  if (hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) {
+    *p = nullptr;
+    return fail(TENTACO_ENOMEM, "batch table allocation failed");
+  }
+  return TENTACO_OK;
+}
+
+void drop_graph(tentaco_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+void free_tables(tentaco_batch* b) {
+  (void)hipFree(b->d_tasks);
+  (void)hipFree(b->d_bnd_off);
+  (void)hipFree(b->d_item_off);
+  (void)hipFree(b->d_brk);
+  b->d_tasks = nullptr;
+  b->d_bnd_off = b->d_item_off = nullptr;
+  b->d_brk = nullptr;
+  b->brk_bytes = 0;
+}
+
+// Rebuild the item/boundary/fill tables and the bracket workspace for a tile
+// configuration, and upload the task table.
+int configure(tentaco_batch* b, int family, int block, int group) {
+  if (family != kEdgeSerial && family != kEdgeParallel)
+    return fail(TENTACO_EINVAL, "batch family must be 0 (edge-serial) or 1 (edge-parallel)");
+  if (block != 128 && block != 256 && block != 512)
+    return fail(TENTACO_EINVAL, "batch block %d not in {128,256,512}", block);
+  if (group < 1 || group > 1024) return fail(TENTACO_EINVAL, "batch group %d", group);
+  if (!b->have_operands) return fail(TENTACO_EINVAL, "set operands before configuring");
+  drop_graph(b);
+  const int lpr = lanes_per_row(b->R);
+  const int64_t NB = int64_t(block / lpr) * group;
+  std::vector<int64_t> bnd(b->ntasks + 1, 0), item(b->ntasks + 1, 0), brk_off(b->ntasks + 1, 0);
+  for (int t = 0; t < b->ntasks; ++t) {
+    const tentaco_csf* h = b->tasks[t];
+    const int64_t nnz = h->nodes[h->order - 1];
+    const int64_t nblk = (nnz + NB - 1) / NB;
+    item[t + 1] = item[t] + nblk;
+    bnd[t + 1] = bnd[t] + nblk + 1;
+    brk_off[t + 1] = brk_off[t] + (nblk + 1) * (h->order - 1);
+  }
+  if (item[b->ntasks] >= (int64_t(1) << 31))
+    return fail(TENTACO_EUNSUPPORTED, "batch has too many work items");
+  free_tables(b);
+  int rc = dev_alloc(&b->d_tasks, size_t(b->ntasks));
+  if (!rc) rc = dev_alloc(&b->d_bnd_off, size_t(b->ntasks));
+  if (!rc) rc = dev_alloc(&b->d_item_off, size_t(b->ntasks));
+  if (!rc) rc = dev_alloc(&b->d_brk, size_t(brk_off[b->ntasks]));
+  if (rc) return rc;
+  for (int t = 0; t < b->ntasks; ++t) b->host_tasks[t].brk = b->d_brk + brk_off[t];
+  TT_HIP_CHECK(hipMemcpy(b->d_tasks, b->host_tasks.data(), sizeof(BatchTaskDev) * b->ntasks,
+                         hipMemcpyHostToDevice));
+  TT_HIP_CHECK(hipMemcpy(b->d_bnd_off, bnd.data(), sizeof(int64_t) * b->ntasks,
+                         hipMemcpyHostToDevice));
+  TT_HIP_CHECK(hipMemcpy(b->d_item_off, item.data(), sizeof(int64_t) * b->ntasks,
+                         hipMemcpyHostToDevice));
+  BatchLaunch& L = b->L;
+  L.tasks = b->d_tasks;
+  L.ntasks = b->ntasks;
+  L.bnd_off = b->d_bnd_off;
+  L.total_bnd = bnd[b->ntasks];
+  L.item_off = b->d_item_off;
+  L.total_items = int32_t(item[b->ntasks]);
+  L.counter = b->d_counter;
+  L.family = family;
+  L.block = block;
+  L.group = group;
+  L.R = b->R;
+  L.NB = int32_t(NB);
+  int per_cu = 1;
+  rc = batch_occupancy(family, block, b->R, &per_cu);
+  if (rc) return rc;
+  int dev = 0;
+  TT_HIP_CHECK(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  TT_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+  L.grid = std::max(1, per_cu) * prop.multiProcessorCount;
+  return TENTACO_OK;
+}
+
+}  // namespace
+}  // namespace tt
+
+using namespace tt;
+
+extern "C" {
+
+int tentaco_batch_create(tentaco_csf* const* tasks, int ntasks, int R, tentaco_batch** out) {
+  clear_error();
+  if (!out || !tasks || ntasks <= 0) return fail(TENTACO_EINVAL, "empty batch");
+  *out = nullptr;
+  if (R != 16 && R != 32 && R != 64)
+    return fail(TENTACO_EUNSUPPORTED, "batched launch supports R in {16,32,64}");
+  for (int t = 0; t < ntasks; ++t)
+    if (!tasks[t]) return fail(TENTACO_EINVAL, "task %d handle is NULL", t);
+  auto* b = new tentaco_batch;
+  b->ntasks = ntasks;
+  b->R = R;
+  b->tasks.assign(tasks, tasks + ntasks);
+  b->host_tasks.resize(size_t(ntasks));
+  if (hipMalloc(&b->d_counter, sizeof(int32_t)) != hipSuccess) {
+    delete b;
+    return fail(TENTACO_ENOMEM, "counter allocation failed");
+  }
+  *out = b;
+  return TENTACO_OK;
+}
+
+int tentaco_batch_set_operands(tentaco_batch* b, const float* const* factors_dev,
+                               float* const* outs_dev, const int32_t* out_rows) {
+  clear_error();
+  if (!b || !factors_dev || !outs_dev || !out_rows) return fail(TENTACO_EINVAL, "NULL argument");
+  std::vector<int64_t> fill(b->ntasks + 1, 0);
+  for (int t = 0; t < b->ntasks; ++t) {
+    const tentaco_csf* h = b->tasks[t];
+    if (out_rows[t] <= h->max_crd[0])
+      return fail(TENTACO_EINVAL, "task %d output has %d rows, root coordinates reach %d", t,
+                  out_rows[t], h->max_crd[0]);
+    if (!outs_dev[t]) return fail(TENTACO_EINVAL, "task %d output is NULL", t);
+    BatchTaskDev& T = b->host_tasks[t];
+    T.B = h->view();
+    for (int m = 1; m < h->order; ++m) {
+      const float* f = factors_dev[size_t(t) * (kMaxOrder - 1) + (m - 1)];
+      if (!f) return fail(TENTACO_EINVAL, "task %d factor of level %d is NULL", t, m);
+      T.F.M[m] = f;
+    }
+    T.A = outs_dev[t];
+    fill[t + 1] = fill[t] + int64_t(out_rows[t]) * b->R;
+  }
+  (void)hipFree(b->d_fill_off);
+  b->d_fill_off = nullptr;
+  int rc = dev_alloc(&b->d_fill_off, size_t(b->ntasks));
+  if (rc) return rc;
+  TT_HIP_CHECK(hipMemcpy(b->d_fill_off, fill.data(), sizeof(int64_t) * b->ntasks,
+                         hipMemcpyHostToDevice));
+  b->L.fill_off = b->d_fill_off;
+  b->L.total_fill = fill[b->ntasks];
+  b->have_operands = true;
+  const bool configured = b->L.NB != 0;
+  return configure(b, configured ? b->L.family : kEdgeParallel, configured ? b->L.block : 256,
+                   configured ? b->L.group : 16);
+}
+
+int tentaco_batch_configure(tentaco_batch* b, int family, int block, int group) {
+  clear_error();
+  if (!b) return fail(TENTACO_EINVAL, "batch is NULL");
+  return configure(b, family, block, group);
+}
+
+int tentaco_batch_run(tentaco_batch* b, void* stream) {
+  if (!b || !b->have_operands) return fail(TENTACO_EINVAL, "batch has no operands");
+  return launch_batch(b->L, static_cast<hipStream_t>(stream));
+}
+
+int tentaco_batch_capture(tentaco_batch* b, void* stream) {
+  clear_error();
+  if (!b || !b->have_operands) return fail(TENTACO_EINVAL, "batch has no operands");
+  drop_graph(b);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  bool own = false;
+  if (!st) {
+    TT_HIP_CHECK(hipStreamCreate(&st));  // capture needs a non-default stream
+    own = true;
+  }
+  TT_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  int rc = launch_batch(b->L, st);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(st, &g);
+  if (own) (void)hipStreamDestroy(st);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(TENTACO_EHIP, "graph capture failed: %s", hipGetErrorString(e));
+  b->graph = g;
+  TT_HIP_CHECK(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+  return TENTACO_OK;
+}
+
+int tentaco_batch_launch(tentaco_batch* b, void* stream) {
+  if (!b || !b->exec) return fail(TENTACO_EINVAL, "batch graph not captured");
+  TT_HIP_CHECK(hipGraphLaunch(b->exec, static_cast<hipStream_t>(stream)));
+  return TENTACO_OK;
+}
+
+int tentaco_batch_items(const tentaco_batch* b, int32_t* items, int32_t* grid, int32_t* nb) {
+  if (!b) return fail(TENTACO_EINVAL, "batch is NULL");
+  if (items) *items = b->L.total_items;
+  if (grid) *grid = b->L.grid;
+  if (nb) *nb = b->L.NB;
+  return TENTACO_OK;
+}
+
+int tentaco_batch_autotune(tentaco_batch* b, int reps, void* stream, int* best_family,
+                           int* best_block, int* best_group, float* best_ms) {
+  clear_error();
+  if (!b || !b->have_operands) return fail(TENTACO_EINVAL, "batch has no operands");
+  if (reps < 1) reps = 1;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  hipEvent_t e0, e1;
+  TT_HIP_CHECK(hipEventCreate(&e0));
+  TT_HIP_CHECK(hipEventCreate(&e1));
+  float bm = -1.f;
+  int bf = -1, bb = 0, bg = 0;
+  std::vector<float> t(static_cast<size_t>(reps));
+  for (int fam = 0; fam < 2; ++fam)
+    for (int blk : {128, 256, 512})
+      for (int grp : {4, 8, 16, 32, 64}) {
+        int rc = tentaco_batch_configure(b, fam, blk, grp);
+        if (!rc) rc = tentaco_batch_run(b, stream);  // warm-up
+        for (int r = 0; r < reps && !rc; ++r) {
+          (void)hipEventRecord(e0, st);
+          rc = tentaco_batch_run(b, stream);
+          (void)hipEventRecord(e1, st);
+          (void)hipEventSynchronize(e1);
+          (void)hipEventElapsedTime(&t[size_t(r)], e0, e1);
+        }
+        if (rc == TENTACO_EUNSUPPORTED) {
+          clear_error();
+          continue;
+        }
+        if (rc) {
+          (void)hipEventDestroy(e0);
+          (void)hipEventDestroy(e1);
+          return rc;
+        }
+        std::sort(t.begin(), t.end());
+        const float med = t[t.size() / 2];
+        if (bf < 0 || med < bm) {
+          bm = med;
+          bf = fam;
+          bb = blk;
+          bg = grp;
+        }
+      }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (bf < 0) return fail(TENTACO_EUNSUPPORTED, "no batch configuration ran");
+  int rc = tentaco_batch_configure(b, bf, bb, bg);
+  if (rc) return rc;
+  if (best_family) *best_family = bf;
+  if (best_block) *best_block = bb;
+  if (best_group) *best_group = bg;
+  if (best_ms) *best_ms = bm;
+  return TENTACO_OK;
+}
+
+int tentaco_batch_free(tentaco_batch* b) {
+  if (!b) return TENTACO_OK;
+  drop_graph(b);
+  free_tables(b);
+  (void)hipFree(b->d_counter);
+  (void)hipFree(b->d_fill_off);
+  delete b;
+  return TENTACO_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,36 @@
+// Device-resident CSF handle (the opaque tentaco_csf of the C ABI).
+#pragma once
+
+#include "internal.h"
+#include "tuning.h"
+
+struct tentaco_csf {
+  int order = 0;
+  int32_t dims[tt::kMaxOrder] = {};
+  int32_t nodes[tt::kMaxOrder] = {};
+  int32_t max_crd[tt::kMaxOrder] = {};  // largest coordinate per level, -1 if none
+  bool dense_root = false;
+  int32_t* d_crd0 = nullptr;
+  int32_t* d_pos[tt::kMaxOrder] = {};
+  int32_t* d_crd[tt::kMaxOrder] = {};
+  float* d_vals = nullptr;
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+  tt::TuneTable tune;  // per-rank best variant found by the autotuner
+  tt::CsfStats stats;
+
+  tt::CsfDev view() const {
+    tt::CsfDev v;
+    v.order = order;
+    v.nnz = nodes[order - 1];
+    for (int m = 0; m < order; ++m) v.nodes[m] = nodes[m];
+    v.crd0 = d_crd0;
+    for (int m = 1; m < order; ++m) {
+      v.pos[m] = d_pos[m];
+      v.crd[m] = d_crd[m];
+    }
+    v.vals = d_vals;
+    return v;
+  }
+};
+

@@ -251,13 +251,21 @@ int build_from_coo(int order, const int32_t* dims, int64_t n, const int32_t* coo
       key.swap(ktmp);
     }
   } else {
-    std::stable_sort(perm.begin(), perm.end(), [&](int64_t a, int64_t b) {
-      for (int m = 0; m < order; ++m) {
-        const int32_t ca = coords[m * n + a], cb = coords[m * n + b];
-        if (ca != cb) return ca < cb;
+    // LSD radix sort over the modes (last mode first), 16-bit digits, stable
+    std::vector<int64_t> tmp(static_cast<size_t>(n));
+    for (int m = order - 1; m >= 0; --m) {
+      const int32_t* cm = coords + int64_t(m) * n;
+      for (int shift = 0; shift < bits[m]; shift += 16) {
+        std::vector<int64_t> cnt(65537, 0);
+        for (int64_t q = 0; q < n; ++q) ++cnt[((uint32_t(cm[perm[size_t(q)]]) >> shift) & 0xFFFF) + 1];
+        for (int i = 0; i < 65536; ++i) cnt[i + 1] += cnt[i];
+        for (int64_t q = 0; q < n; ++q) {
+          const int64_t p = perm[size_t(q)];
+          tmp[size_t(cnt[(uint32_t(cm[p]) >> shift) & 0xFFFF]++)] = p;
+        }
+        perm.swap(tmp);
       }
-      return false;
-    });
+    }
   }
   HostCsf& h = *out;
   h.order = order;
@@ -495,6 +503,48 @@ int tentaco_synth_csf(const tentaco_synth_params* p, tentaco_host_csf** out) {
   }
   *out = h;
   return TENTACO_OK;
+}
+
+int tentaco_csf_reorder(const taco_tensor_t* B, int order, const int* mode_order,
+                        tentaco_host_csf** out) {
+  tt::clear_error();
+  if (!B || !mode_order || !out) return fail(TENTACO_EINVAL, "NULL argument");
+  *out = nullptr;
+  if (order < 2 || order > tt::kMaxOrder)
+    return fail(TENTACO_EUNSUPPORTED, "order %d outside 2..5", order);
+  bool seen[tt::kMaxOrder] = {};
+  for (int m = 0; m < order; ++m) {
+    const int src = mode_order[m];
+    if (src < 0 || src >= order || seen[src]) return fail(TENTACO_EINVAL, "mode_order is not a permutation");
+    seen[src] = true;
+  }
+  // expand to coordinates (level order of B), then permute the modes
+  int64_t nodes[tt::kMaxOrder];
+  nodes[0] = B->indices[0][0][1];
+  for (int m = 1; m < order; ++m) nodes[m] = B->indices[m][0][nodes[m - 1]];
+  const int64_t nnz = nodes[order - 1];
+  std::vector<int32_t> coords(static_cast<size_t>(nnz) * order);
+  std::vector<int64_t> node(static_cast<size_t>(nnz));
+  for (int64_t q = 0; q < nnz; ++q) node[size_t(q)] = q;
+  for (int m = order - 1; m >= 0; --m) {
+    const int32_t* crd = B->indices[m][1];
+    for (int64_t q = 0; q < nnz; ++q) coords[size_t(m) * nnz + q] = crd[node[size_t(q)]];
+    if (m == 0) break;
+    // parent of each node: walk the pos array once (nodes are in order)
+    const int32_t* pos = B->indices[m][0];
+    std::vector<int64_t> parent_of(static_cast<size_t>(nodes[m]));
+    for (int64_t x = 0; x < nodes[m - 1]; ++x)
+      for (int64_t c = pos[x]; c < pos[x + 1]; ++c) parent_of[size_t(c)] = x;
+    for (int64_t q = 0; q < nnz; ++q) node[size_t(q)] = parent_of[size_t(node[size_t(q)])];
+  }
+  std::vector<int32_t> pc(coords.size());
+  int32_t pdims[tt::kMaxOrder];
+  for (int m = 0; m < order; ++m) {
+    pdims[m] = B->dimensions[mode_order[m]];
+    std::memcpy(pc.data() + size_t(m) * nnz, coords.data() + size_t(mode_order[m]) * nnz,
+                sizeof(int32_t) * size_t(nnz));
+  }
+  return tentaco_coo_to_csf(order, pdims, nnz, pc.data(), B->vals, out);
 }
 
 taco_tensor_t* tentaco_host_csf_view(tentaco_host_csf* h) { return h ? &h->h.view : nullptr; }

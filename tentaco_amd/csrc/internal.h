@@ -70,6 +70,33 @@ int launch_mttkrp(const CsfDev& B, const FactorsDev& F, int R, float* A,
 int launch_ttv(const CsfDev& B, const float* c, float* y, void* ws,
                hipStream_t stream);
 
+// One task of a batched persistent launch (a CSF with its factors, output
+// and bracket workspace).
+struct BatchTaskDev {
+  CsfDev B;
+  FactorsDev F;
+  float* A = nullptr;
+  int32_t* brk = nullptr;
+};
+
+struct BatchLaunch {
+  const BatchTaskDev* tasks = nullptr;  // device
+  int32_t ntasks = 0;
+  const int64_t* fill_off = nullptr;    // device, prefix of output sizes (floats)
+  int64_t total_fill = 0;
+  const int64_t* bnd_off = nullptr;     // device, prefix of (items+1) per task
+  int64_t total_bnd = 0;
+  const int64_t* item_off = nullptr;    // device, prefix of items per task
+  int32_t total_items = 0;
+  int32_t* counter = nullptr;           // device work counter
+  int family = 1, block = 256, group = 16, R = 32;
+  int32_t NB = 0;
+  int grid = 0;
+};
+
+int launch_batch(const BatchLaunch& L, hipStream_t stream);
+int batch_occupancy(int family, int block, int R, int* per_cu);
+
 // Fill n floats with value on `stream` (kernel, not hipMemsetAsync, so it is
 // safe inside captured graphs).
 int launch_fill(float* p, int64_t n, float value, hipStream_t stream);

@@ -22,9 +22,9 @@
 //    parallel reduction in LDS (the role of segReduceGroup,
 //    gpu_library.h:295-321, redone for wave64 and R-wide rows); only the
 //    workgroup's first and last slice are added atomically.
-//  * slice-row (SR): one wavefront per root node, its level-1 children
-//    spread over the row groups, rows reduced across the wave with xor
-//    shuffles and stored once. No atomics, no zero fill for a dense root,
+//  * slice-row (SR): one workgroup per root node, its level-1 children
+//    spread over the workgroup's row groups, rows reduced with xor shuffles
+//    and LDS and stored once. No atomics, no zero fill for a dense root,
 //    deterministic; balanced only when slices are.
 //
 // Runs find their starting node at every level with an upper-bound binary
@@ -73,13 +73,14 @@ __global__ __launch_bounds__(256) void edge_prep_kernel(CsfDev B, float* A,
 }
 
 // ---------------------------------------------------------------------------
-// edge-serial / edge-parallel kernel.
+// edge-serial / edge-parallel work for workgroup-sized block b of one CSF
+// (nonzeros [b*NB, (b+1)*NB)); shared by the per-tensor kernel and the
+// persistent batch kernel. `smem` is the EP reduction area (dynamic LDS).
 template <int N, int R, bool EP>
-__global__ __launch_bounds__(512) void edge_kernel(CsfDev B, FactorsDev F,
-                                                   float* __restrict__ A,
-                                                   int Rr,
-                                                   const int32_t* __restrict__ brk,
-                                                   int32_t G, int32_t NB) {
+__device__ __forceinline__ void edge_block(const CsfDev& B, const FactorsDev& F,
+                                           float* __restrict__ A, int Rr,
+                                           const int32_t* __restrict__ brk, int32_t b,
+                                           int32_t G, int32_t NB, unsigned char* smem) {
   using RO = Row<R>;
   using V = typename RO::V;
   constexpr int NV = RO::NV;
@@ -88,7 +89,6 @@ __global__ __launch_bounds__(512) void edge_kernel(CsfDev B, FactorsDev F,
 
   const int gi = threadIdx.x / LPR;
   const int c = threadIdx.x % LPR;
-  const int32_t b = blockIdx.x;
   const int32_t nnz = B.nnz;
   const int32_t blk0 = b * NB;
   const int32_t blk1 = min(blk0 + NB, nnz);
@@ -206,7 +206,6 @@ __global__ __launch_bounds__(512) void edge_kernel(CsfDev B, FactorsDev F,
     // last slice) of the workgroup; keys are nondecreasing, idle groups use
     // INT_MAX. Entry 2g = (head slice, its closed partial or 0), entry
     // 2g+1 = (open slice at run end, its partial).
-    extern __shared__ __align__(16) unsigned char smem[];
     const int NGB = blockDim.x / LPR;
     const int E = 2 * NGB;
     const int Rc = (R == 0) ? Rr : R;
@@ -286,6 +285,15 @@ __global__ __launch_bounds__(512) void edge_kernel(CsfDev B, FactorsDev F,
   }
 }
 
+template <int N, int R, bool EP>
+__global__ __launch_bounds__(512) void edge_kernel(CsfDev B, FactorsDev F,
+                                                   float* __restrict__ A, int Rr,
+                                                   const int32_t* __restrict__ brk,
+                                                   int32_t G, int32_t NB) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  edge_block<N, R, EP>(B, F, A, Rr, brk, blockIdx.x, G, NB, smem);
+}
+
 // ---------------------------------------------------------------------------
 // slice-row kernel: subtree sums, depth first.
 template <int N, int M, int R>
@@ -338,6 +346,91 @@ __device__ __forceinline__ void subtree(const CsfDev& B, const FactorsDev& F,
   }
 }
 
+// Order-3 slice: the wavefront walks the slice's level-1 fibers 64 at a time.
+// Lane L loads fiber (base+L)'s bounds, coordinate and first nonzero with
+// coalesced loads; row group g then takes fibers base + t*NG + g through
+// lane shuffles and issues all its C and D row gathers for the chunk before
+// using any of them. The next chunk's fiber metadata is loaded while the
+// current chunk's rows are in flight.
+template <int R>
+__device__ __forceinline__ void slice3_accumulate(const CsfDev& B, const FactorsDev& F,
+                                                  int32_t x, int lane, int gi, int c, int Rr,
+                                                  int wv, int nwv,
+                                                  typename Row<R>::V (&acc)[Row<R>::NV]) {
+  using RO = Row<R>;
+  using V = typename RO::V;
+  constexpr int NV = RO::NV;
+  constexpr int NG = 64 / RO::LPR;
+  constexpr int NT = 64 / NG;                 // fibers per row group per chunk
+  constexpr int TB = NT < 8 ? NT : 8;         // fibers whose rows are in flight per batch
+  const int32_t* __restrict__ p2 = B.pos[2];
+  const int32_t* __restrict__ c1 = B.crd[1];
+  const int32_t* __restrict__ c2 = B.crd[2];
+  const float* __restrict__ vals = B.vals;
+  const float* __restrict__ Cm = F.M[1];
+  const float* __restrict__ Dm = F.M[2];
+  const int32_t f0 = B.pos[1][x];
+  const int32_t f1 = B.pos[1][x + 1];
+  auto meta = [&](int32_t base, int32_t& s, int32_t& e, int32_t& k, int32_t& l0, float& v0) {
+    const int32_t f = base + lane;
+    s = e = k = l0 = 0;
+    v0 = 0.f;
+    if (f < f1) {
+      s = p2[f];
+      e = p2[f + 1];
+      k = c1[f];
+      if (e > s) {
+        l0 = c2[s];
+        v0 = vals[s];
+      }
+    }
+  };
+  const int32_t step = 64 * nwv;
+  int32_t ns, ne, nk, nl;
+  float nv;
+  if (f0 + 64 * wv < f1) meta(f0 + 64 * wv, ns, ne, nk, nl, nv);
+  for (int32_t base = f0 + 64 * wv; base < f1; base += step) {
+    const int32_t s = ns, e = ne, k = nk, l0 = nl;
+    const float v0 = nv;
+    if (base + step < f1) meta(base + step, ns, ne, nk, nl, nv);
+#pragma unroll
+    for (int t0 = 0; t0 < NT; t0 += TB) {
+      V cr[TB][NV], dr[TB][NV];
+      int32_t st[TB], et[TB];
+      float vt[TB];
+      bool ok[TB];
+#pragma unroll
+      for (int u = 0; u < TB; ++u) {
+        const int src = (t0 + u) * NG + gi;
+        const int32_t kt = __shfl(k, src);
+        const int32_t lt = __shfl(l0, src);
+        vt[u] = __shfl(v0, src);
+        st[u] = __shfl(s, src);
+        et[u] = __shfl(e, src);
+        ok[u] = base + src < f1;
+        RO::load(cr[u], Cm, ok[u] ? kt : 0, c, Rr);
+        RO::load(dr[u], Dm, ok[u] ? lt : 0, c, Rr);
+      }
+#pragma unroll
+      for (int u = 0; u < TB; ++u) {
+        if (!ok[u]) continue;
+        V w[NV];
+        RO::zero(w);
+        RO::axpy(w, vt[u], dr[u]);
+        for (int32_t q = st[u] + 1; q < et[u]; ++q) {  // fibers longer than one nonzero
+          V d[NV];
+          RO::load(d, Dm, c2[q], c, Rr);
+          RO::axpy(w, vals[q], d);
+        }
+        RO::hfma(acc, w, cr[u]);
+      }
+    }
+  }
+}
+
+// One workgroup per root node; its blockDim/64 wavefronts split the slice
+// (64-child chunks round-robin), reduce their rows with xor shuffles, then
+// across wavefronts through LDS; one store per row, no atomics.
 template <int N, int R>
 __global__ __launch_bounds__(512) void slice_kernel(CsfDev B, FactorsDev F,
                                                     float* __restrict__ A,
@@ -347,32 +440,81 @@ __global__ __launch_bounds__(512) void slice_kernel(CsfDev B, FactorsDev F,
   constexpr int NV = RO::NV;
   constexpr int LPR = RO::LPR;
   constexpr int NG = 64 / LPR;
-  const int64_t wave = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
-  if (wave >= B.nodes[0]) return;  // whole wavefront exits together
+  __shared__ __align__(16) float red[8][Lay<R>::kMaxRank];
+  const int32_t x = blockIdx.x;
   const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int nwv = blockDim.x >> 6;
   const int gi = lane / LPR;
   const int c = lane % LPR;
-  const int32_t x = int32_t(wave);
   V acc[NV];
   RO::zero(acc);
   const int32_t a = B.pos[1][x];
   const int32_t e = B.pos[1][x + 1];
   if constexpr (N == 2) {
-    // children are nonzeros: row groups take them round-robin
-    for (int32_t y = a + gi; y < e; y += NG) {
-      V d[NV];
-      RO::load(d, F.M[1], B.crd[1][y], c, Rr);
-      RO::axpy(acc, B.vals[y], d);
+    // children are nonzeros: lanes load 64 at a time, row groups take them
+    // through shuffles with every gather of a chunk in flight at once
+    constexpr int NT = 64 / NG;
+    for (int32_t base = a + 64 * wv; base < e; base += 64 * nwv) {
+      const int32_t q = base + lane;
+      const int32_t lq = q < e ? B.crd[1][q] : 0;
+      const float vq = q < e ? B.vals[q] : 0.f;
+      V d[NT][NV];
+      float vt[NT];
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const int src = t * NG + gi;
+        vt[t] = __shfl(vq, src);
+        RO::load(d[t], F.M[1], __shfl(lq, src), c, Rr);
+      }
+#pragma unroll
+      for (int t = 0; t < NT; ++t) RO::axpy(acc, vt[t], d[t]);
     }
+  } else if constexpr (N == 3) {
+    slice3_accumulate<R>(B, F, x, lane, gi, c, Rr, wv, nwv, acc);
   } else {
-    for (int32_t y = a + gi; y < e; y += NG) {
+    for (int32_t y = a + wv * NG + gi; y < e; y += NG * nwv) {
       V s[NV];
       subtree<N, 1, R>(B, F, y, c, Rr, s);
       RO::add(acc, s);
     }
   }
   RO::wave_sum(acc);
-  if (gi == 0) RO::store(A, B.crd0[x], c, Rr, acc);
+  if (nwv == 1) {
+    if (gi == 0) RO::store(A, B.crd0[x], c, Rr, acc);
+    return;
+  }
+  const int Rc = (R == 0) ? Rr : R;
+  if (gi == 0) {
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      const int k = RO::col(j, c);
+      if constexpr (R == 0) {
+        if (k < Rc) red[wv][k] = acc[j];
+      } else {
+        *reinterpret_cast<float4*>(&red[wv][k]) = acc[j];
+      }
+    }
+  }
+  __syncthreads();
+  if (wv == 0 && gi == 0) {
+    for (int w = 1; w < nwv; ++w) {
+#pragma unroll
+      for (int j = 0; j < NV; ++j) {
+        const int k = RO::col(j, c);
+        if constexpr (R == 0) {
+          if (k < Rc) acc[j] += red[w][k];
+        } else {
+          const float4 o = *reinterpret_cast<const float4*>(&red[w][k]);
+          acc[j].x += o.x;
+          acc[j].y += o.y;
+          acc[j].z += o.z;
+          acc[j].w += o.w;
+        }
+      }
+    }
+    RO::store(A, B.crd0[x], c, Rr, acc);
+  }
 }
 
 __global__ __launch_bounds__(256) void fill_kernel(float* A, int64_t n, float value) {
@@ -383,6 +525,128 @@ __global__ __launch_bounds__(256) void fill_kernel(float* A, int64_t n, float va
   const float4 v4 = make_float4(value, value, value, value);
   for (int64_t i = tid; i < n4; i += nthr) A4[i] = v4;
   for (int64_t i = (n4 << 2) + tid; i < n; i += nthr) A[i] = value;
+}
+
+int grid_for(int64_t work, int block, int cap);
+
+// ---------------------------------------------------------------------------
+// batched persistent launch: every (tensor, mode) task's nonzeros are cut into
+// the same NB-sized items; persistent workgroups pull items from a device
+// counter (reset by the batch prep launch) until the queue is empty, and run
+// the edge walk of the item's task. Every workgroup leaves the loop when the
+// counter passes the item count.
+__device__ __forceinline__ int32_t upper_index(const int64_t* __restrict__ off, int32_t n,
+                                               int64_t key) {
+  // largest t in [0, n) with off[t] <= key
+  int32_t lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int32_t mid = lo + ((hi - lo + 1) >> 1);
+    if (off[mid] <= key) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void batch_prep_kernel(const BatchTaskDev* __restrict__ tasks,
+                                                         int32_t ntasks,
+                                                         const int64_t* __restrict__ fill_off,
+                                                         int64_t total_fill,
+                                                         const int64_t* __restrict__ bnd_off,
+                                                         int64_t total_bnd,
+                                                         int32_t* __restrict__ counter,
+                                                         int32_t NB) {
+  const int64_t tid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t nthr = int64_t(gridDim.x) * blockDim.x;
+  if (tid == 0) *counter = 0;
+  for (int64_t i = tid; i < total_fill; i += nthr) {
+    const int32_t t = upper_index(fill_off, ntasks, i);
+    tasks[t].A[i - fill_off[t]] = 0.f;
+  }
+  for (int64_t i = tid; i < total_bnd; i += nthr) {
+    const int32_t t = upper_index(bnd_off, ntasks, i);
+    const BatchTaskDev& T = tasks[t];
+    if (T.B.nnz <= 0) continue;
+    const int64_t bi = i - bnd_off[t];
+    int64_t p = bi * int64_t(NB);
+    if (p > T.B.nnz - 1) p = T.B.nnz - 1;
+    int32_t key = int32_t(p);
+    const int N = T.B.order;
+    for (int m = N - 2; m >= 0; --m) {
+      const int32_t x = search_le(T.B.pos[m + 1], 0, T.B.nodes[m] - 1, key);
+      T.brk[bi * (N - 1) + m] = x;
+      key = x;
+    }
+  }
+}
+
+template <int R, bool EP>
+__global__ __launch_bounds__(512) void batch_kernel(const BatchTaskDev* __restrict__ tasks,
+                                                    int32_t ntasks,
+                                                    const int64_t* __restrict__ item_off,
+                                                    int32_t total_items,
+                                                    int32_t* __restrict__ counter, int Rr,
+                                                    int32_t G, int32_t NB) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ int32_t s_item;
+  for (;;) {
+    __syncthreads();  // the previous item's LDS reads are done
+    if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+    __syncthreads();
+    const int32_t item = __builtin_amdgcn_readfirstlane(s_item);
+    if (item >= total_items) break;
+    const int32_t t = upper_index(item_off, ntasks, item);
+    const BatchTaskDev& T = tasks[t];
+    const int32_t b = item - int32_t(item_off[t]);
+    switch (T.B.order) {
+      case 2: edge_block<2, R, EP>(T.B, T.F, T.A, Rr, T.brk, b, G, NB, smem); break;
+      case 3: edge_block<3, R, EP>(T.B, T.F, T.A, Rr, T.brk, b, G, NB, smem); break;
+      case 4: edge_block<4, R, EP>(T.B, T.F, T.A, Rr, T.brk, b, G, NB, smem); break;
+      default: edge_block<5, R, EP>(T.B, T.F, T.A, Rr, T.brk, b, G, NB, smem); break;
+    }
+  }
+}
+
+template <int R>
+int launch_batch_typed(const BatchLaunch& L, hipStream_t st) {
+  const int lpr = Row<R>::LPR;
+  size_t lds = 0;
+  if (L.family == kEdgeParallel) {
+    const int E = 2 * (L.block / lpr);
+    lds = size_t(E) * sizeof(int32_t) + size_t(E) * L.R * sizeof(float);
+    if (lds > 65536) return fail(TENTACO_EUNSUPPORTED, "batch LDS %zu B", lds);
+  }
+  {
+    const int64_t work = std::max<int64_t>(L.total_fill, L.total_bnd);
+    batch_prep_kernel<<<grid_for(work, 256, 16384), 256, 0, st>>>(
+        L.tasks, L.ntasks, L.fill_off, L.total_fill, L.bnd_off, L.total_bnd, L.counter, L.NB);
+    TT_HIP_CHECK(hipGetLastError());
+  }
+  if (L.total_items == 0) return TENTACO_OK;
+  const int grid = std::max(1, std::min<int>(L.grid, L.total_items));
+  if (L.family == kEdgeParallel)
+    batch_kernel<R, true><<<grid, L.block, lds, st>>>(L.tasks, L.ntasks, L.item_off,
+                                                       L.total_items, L.counter, L.R, L.group, L.NB);
+  else
+    batch_kernel<R, false><<<grid, L.block, 0, st>>>(L.tasks, L.ntasks, L.item_off,
+                                                      L.total_items, L.counter, L.R, L.group, L.NB);
+  TT_HIP_CHECK(hipGetLastError());
+  return TENTACO_OK;
+}
+
+template <int R>
+int batch_occupancy_typed(int family, int block, int Rr, int* per_cu) {
+  const int lpr = Row<R>::LPR;
+  size_t lds = 0;
+  hipError_t e;
+  if (family == kEdgeParallel) {
+    const int E = 2 * (block / lpr);
+    lds = size_t(E) * sizeof(int32_t) + size_t(E) * Rr * sizeof(float);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, batch_kernel<R, true>, block, lds);
+  } else {
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, batch_kernel<R, false>, block, 0);
+  }
+  if (e != hipSuccess) return fail(TENTACO_EHIP, "occupancy query: %s", hipGetErrorString(e));
+  return TENTACO_OK;
 }
 
 int grid_for(int64_t work, int block, int cap) {
@@ -403,9 +667,7 @@ int launch_typed(const CsfDev& B, const FactorsDev& F, int Rr, float* A,
       TT_HIP_CHECK(hipGetLastError());
     }
     if (B.nodes[0] > 0) {
-      const int64_t threads = int64_t(B.nodes[0]) * 64;
-      const int grid = int((threads + vr.block - 1) / vr.block);
-      slice_kernel<N, R><<<grid, vr.block, 0, st>>>(B, F, A, Rr);
+      slice_kernel<N, R><<<B.nodes[0], vr.block, 0, st>>>(B, F, A, Rr);
       TT_HIP_CHECK(hipGetLastError());
     }
     return TENTACO_OK;
@@ -458,6 +720,24 @@ int launch_fill(float* p, int64_t n, float value, hipStream_t st) {
   fill_kernel<<<grid_for(n / 4 + 1, 256, 16384), 256, 0, st>>>(p, n, value);
   TT_HIP_CHECK(hipGetLastError());
   return TENTACO_OK;
+}
+
+int launch_batch(const BatchLaunch& L, hipStream_t st) {
+  switch (L.R) {
+    case 16: return launch_batch_typed<16>(L, st);
+    case 32: return launch_batch_typed<32>(L, st);
+    case 64: return launch_batch_typed<64>(L, st);
+    default: return fail(TENTACO_EUNSUPPORTED, "batched launch supports R in {16,32,64}");
+  }
+}
+
+int batch_occupancy(int family, int block, int R, int* per_cu) {
+  switch (R) {
+    case 16: return batch_occupancy_typed<16>(family, block, R, per_cu);
+    case 32: return batch_occupancy_typed<32>(family, block, R, per_cu);
+    case 64: return batch_occupancy_typed<64>(family, block, R, per_cu);
+    default: return fail(TENTACO_EUNSUPPORTED, "batched launch supports R in {16,32,64}");
+  }
 }
 
 Variant variant_of(int v) {

@@ -48,6 +48,10 @@ __global__ __launch_bounds__(kTtvBlock) void ttv_kernel(
     const float* __restrict__ vals, const float* __restrict__ cv,
     float* __restrict__ y, int32_t nnz, const int32_t* __restrict__ brk) {
   constexpr int U = kTtvPerLane;
+  // fiber bounds pos[lo..hi+1] of the workgroup, staged once; a CSF built
+  // from coordinates has at most NB+1 non-empty fibers starting in a
+  // workgroup, larger brackets (runs of empty fibers) read pos directly
+  __shared__ int32_t s_pos[kTtvNB + 2];
   const int32_t b = blockIdx.x;
   const int32_t P = b * kTtvNB;
   const int32_t Pend = min(P + kTtvNB, nnz);
@@ -55,16 +59,27 @@ __global__ __launch_bounds__(kTtvBlock) void ttv_kernel(
   const int32_t p0 = P + int32_t(threadIdx.x) * U;
   const int32_t p1 = min(p0 + U, Pend);
   const bool active = p0 < p1;
+  const int32_t lo = brk[b];
   const int32_t hi = brk[b + 1];
+  const int32_t nstage = hi - lo + 2;
+  const bool staged = nstage <= kTtvNB + 2;
+  if (staged)
+    for (int32_t j = threadIdx.x; j < nstage; j += kTtvBlock) s_pos[j] = pos[lo + j];
+  __syncthreads();
+  auto P_at = [&](int32_t x) -> int32_t { return staged ? s_pos[x - lo] : pos[x]; };
+  auto search = [&](int32_t a, int32_t z, int32_t key) {
+    while (a < z) {
+      const int32_t mid = a + ((z - a + 1) >> 1);
+      if (P_at(mid) <= key) a = mid;
+      else z = mid - 1;
+    }
+    return a;
+  };
 
   int32_t f = INT_MAX, nxt = 0, first = INT_MAX;
   float sum = 0.f, vf = 0.f;
   bool closed_first = false, ends_here = false, starts_before = false;
   if (active) {
-    f = search_le(pos, brk[b], hi, p0);
-    nxt = pos[f + 1];
-    first = f;
-    starts_before = pos[f] < p0;
     int32_t k[U];
     float v[U];
     if (p0 + U <= nnz) {
@@ -83,6 +98,10 @@ __global__ __launch_bounds__(kTtvBlock) void ttv_kernel(
     float cg[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) cg[u] = cv[k[u]];
+    f = search(lo, hi, p0);
+    nxt = P_at(f + 1);
+    first = f;
+    starts_before = P_at(f) < p0;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int32_t p = p0 + u;
@@ -96,10 +115,10 @@ __global__ __launch_bounds__(kTtvBlock) void ttv_kernel(
         }
         sum = 0.f;
         int32_t x = f + 1;
-        int32_t e = pos[x + 1];
+        int32_t e = P_at(x + 1);
         if (e <= p) {
-          x = search_le(pos, x + 1, hi, p);
-          e = pos[x + 1];
+          x = search(x + 1, hi, p);
+          e = P_at(x + 1);
         }
         f = x;
         nxt = e;

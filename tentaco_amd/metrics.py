@@ -85,3 +85,22 @@ def workload_zipf(order, dim, nnz, seed, slice_s=1.1, leaf_sigma=1.0):
 def factors(B, R, seed):
     rng = np.random.default_rng(seed)
     return [rng.uniform(-1, 1, (B.dims[m], R)).astype(np.float32) for m in range(1, B.order)]
+
+
+def workload_batch(ntensors=64, seed=1003, nnz_lo=1e4, nnz_hi=5e6, dim_lo=1e3, dim_hi=1e6):
+    """Uneven batch: order U{3,4,5}, per-mode dims log-U[1e3,1e6], nnz
+    log-U[1e4,5e6], Zipf slice exponent U[0.8,1.5]; returns the tensors (one
+    CSF each, natural mode order)."""
+    import tentaco_amd as tt
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for t in range(ntensors):
+        order = int(rng.integers(3, 6))
+        dims = [int(round(10 ** rng.uniform(np.log10(dim_lo), np.log10(dim_hi)))) for _ in range(order)]
+        nnz = int(round(10 ** rng.uniform(np.log10(nnz_lo), np.log10(nnz_hi))))
+        s = float(rng.uniform(0.8, 1.5))
+        out.append(tt.Csf.synth(order, dims, nnz, slice_s=s, fanout_s=1.3, mid_mu=1.0,
+                                mid_sigma=1.0, leaf_mu=1.5, leaf_sigma=1.0, leaf_cap=10000,
+                                seed=seed * 1000 + t))
+    return out
