@@ -110,7 +110,7 @@ int configure(tentaco_batch* b, int family, int block, int group) {
   L.R = b->R;
   L.NB = int32_t(NB);
   int per_cu = 1;
-  rc = batch_occupancy(family, block, b->R, &per_cu);
+  rc = batch_occupancy(family, block, b->R, int32_t(NB), &per_cu);
   if (rc) return rc;
   int dev = 0;
   TT_HIP_CHECK(hipGetDevice(&dev));

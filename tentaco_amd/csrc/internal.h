@@ -95,7 +95,7 @@ struct BatchLaunch {
 };
 
 int launch_batch(const BatchLaunch& L, hipStream_t stream);
-int batch_occupancy(int family, int block, int R, int* per_cu);
+int batch_occupancy(int family, int block, int R, int32_t NB, int* per_cu);
 
 // Fill n floats with value on `stream` (kernel, not hipMemsetAsync, so it is
 // safe inside captured graphs).

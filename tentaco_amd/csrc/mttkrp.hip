@@ -75,7 +75,34 @@ __global__ __launch_bounds__(256) void edge_prep_kernel(CsfDev B, float* A,
 // ---------------------------------------------------------------------------
 // edge-serial / edge-parallel work for workgroup-sized block b of one CSF
 // (nonzeros [b*NB, (b+1)*NB)); shared by the per-tensor kernel and the
-// persistent batch kernel. `smem` is the EP reduction area (dynamic LDS).
+// persistent batch kernel.
+//
+// LDS layout (dynamic, `smem`): the EP reduction area (edge_ep_bytes), then
+// for each walk level k = 0..N-2 the node bounds pos[k+1][lo..hi+1] and the
+// node coordinates crd[k][lo..hi] of the workgroup's bracket [lo, hi]. The
+// level above the nonzeros (k = N-2) gets room for min(NB+1, kLeafCap)
+// nodes (NB+1 is the most a CSF built from coordinates can have there); the
+// upper levels get kUpperCap. A level whose bracket does not fit is read
+// from global memory instead, so large brackets and runs of empty nodes cost
+// time, never correctness.
+constexpr int kUpperCap = 256;
+constexpr int kLeafCap = 1024;
+
+__host__ __device__ inline int32_t edge_leaf_cap(int32_t NB) {
+  return NB + 1 < kLeafCap ? NB + 1 : kLeafCap;
+}
+
+__host__ __device__ inline size_t edge_ep_bytes(int block, int lpr, int Rc) {
+  const size_t E = size_t(2) * (block / lpr);
+  return E * sizeof(int32_t) + E * size_t(Rc) * sizeof(float);
+}
+
+__host__ __device__ inline size_t edge_stage_ints(int order, int32_t NB) {
+  size_t n = size_t(2) * edge_leaf_cap(NB) + 1;  // level order-2
+  n += size_t(order - 2) * (2 * kUpperCap + 1);
+  return n;
+}
+
 template <int N, int R, bool EP>
 __device__ __forceinline__ void edge_block(const CsfDev& B, const FactorsDev& F,
                                            float* __restrict__ A, int Rr,
@@ -95,8 +122,54 @@ __device__ __forceinline__ void edge_block(const CsfDev& B, const FactorsDev& F,
   const int32_t p0 = blk0 + gi * G;
   const int32_t p1 = min(p0 + G, blk1);
   const bool active = p0 < p1;
+  const int Rc = (R == 0) ? Rr : R;
 
-  int32_t cur[NL], cend[NL], hib[NL];
+  // ---- stage the node brackets in LDS
+  int32_t lo[NL], hib[NL];
+  bool staged[NL];
+  int32_t* spos[NL];
+  int32_t* scrd[NL];
+  {
+    int32_t* base = reinterpret_cast<int32_t*>(
+        smem + (EP ? edge_ep_bytes(blockDim.x, LPR, Rc) : size_t(0)));
+#pragma unroll
+    for (int k = NL - 1; k >= 0; --k) {
+      const int32_t cap = (k == NL - 1) ? edge_leaf_cap(NB) : kUpperCap;
+      lo[k] = brk[b * NL + k];
+      hib[k] = brk[(b + 1) * NL + k];
+      staged[k] = hib[k] - lo[k] + 1 <= cap;
+      spos[k] = base;
+      scrd[k] = base + cap + 1;
+      base += 2 * cap + 1;
+      if (staged[k]) {
+        const int32_t* pk = B.pos[k + 1];
+        const int32_t* ck = (k == 0) ? B.crd0 : B.crd[k];
+        const int32_t n = hib[k] - lo[k] + 1;
+        for (int32_t j = threadIdx.x; j <= n; j += blockDim.x) {
+          spos[k][j] = pk[lo[k] + j];
+          if (j < n) scrd[k][j] = ck[lo[k] + j];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  auto POS = [&](int k, int32_t x) -> int32_t {
+    return staged[k] ? spos[k][x - lo[k]] : B.pos[k + 1][x];
+  };
+  auto CRD = [&](int k, int32_t x) -> int32_t {
+    return staged[k] ? scrd[k][x - lo[k]] : ((k == 0) ? B.crd0[x] : B.crd[k][x]);
+  };
+  auto SEARCH = [&](int k, int32_t a, int32_t z, int32_t key) {
+    // largest x in [a, z] with POS(k, x) <= key (requires POS(k, a) <= key)
+    while (a < z) {
+      const int32_t mid = a + ((z - a + 1) >> 1);
+      if (POS(k, mid) <= key) a = mid;
+      else z = mid - 1;
+    }
+    return a;
+  };
+
+  int32_t cur[NL], cend[NL];
   V acc[NL][NV];
   V rowc[NL][NV];
 #pragma unroll
@@ -105,48 +178,52 @@ __device__ __forceinline__ void edge_block(const CsfDev& B, const FactorsDev& F,
     RO::zero(rowc[k]);
     cur[k] = 0;
     cend[k] = 0;
-    hib[k] = 0;
   }
   int32_t head = INT_MAX;
+  int32_t head_row = 0;
   bool head_closed = false;
   V head_acc[NV];
   RO::zero(head_acc);
 
   if (active) {
-    // starting node at each level: bottom-up upper-bound searches inside the
-    // workgroup bracket
     int32_t key = p0;
 #pragma unroll
     for (int k = NL - 1; k >= 0; --k) {
-      const int32_t lo = brk[b * NL + k];
-      hib[k] = brk[(b + 1) * NL + k];
-      cur[k] = search_le(B.pos[k + 1], lo, hib[k], key);
+      cur[k] = SEARCH(k, lo[k], hib[k], key);
       key = cur[k];
     }
 #pragma unroll
     for (int k = 0; k < NL; ++k) {
-      cend[k] = B.pos[k + 1][cur[k] + 1];
-      if (k >= 1) RO::load(rowc[k], F.M[k], B.crd[k][cur[k]], c, Rr);
+      cend[k] = POS(k, cur[k] + 1);
+      if (k >= 1) RO::load(rowc[k], F.M[k], CRD(k, cur[k]), c, Rr);
     }
     head = cur[0];
+    head_row = CRD(0, cur[0]);
 
     const int32_t* __restrict__ lcrd = B.crd[N - 1];
     const float* __restrict__ vals = B.vals;
     const float* __restrict__ Mleaf = F.M[N - 1];
-    constexpr int U = 4;
-    for (int32_t t0 = p0; t0 < p1; t0 += U) {
-      int32_t l[U];
-      float v[U];
-      V d[U][NV];
+    // Leaf (coordinate, value) pairs arrive U at a time: lane c < U of a row
+    // group loads pair t0 + c (one coalesced load per group and batch) and
+    // the group's lanes exchange them with shuffles. Pairs are fetched two
+    // batches ahead and leaf rows one batch ahead of their use.
+    constexpr int U = LPR < 4 ? LPR : 4;
+    const int gbase = (threadIdx.x & 63) - c;  // lane of this group's lane 0
+    auto load_pairs = [&](int32_t t0, int32_t& lq, float& vq) {
+      const int32_t t = t0 + c;
+      const bool ok = c < U && t < p1;
+      lq = ok ? lcrd[t] : 0;
+      vq = ok ? vals[t] : 0.f;
+    };
+    auto issue = [&](int32_t lq, float vq, V (&d)[U][NV], float (&v)[U]) {
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int32_t t = t0 + u;
-        const bool ok = t < p1;
-        l[u] = ok ? lcrd[t] : 0;
-        v[u] = ok ? vals[t] : 0.f;
+        const int32_t l = __shfl(lq, gbase + u);
+        v[u] = __shfl(vq, gbase + u);
+        RO::load(d[u], Mleaf, l, c, Rr);
       }
-#pragma unroll
-      for (int u = 0; u < U; ++u) RO::load(d[u], Mleaf, l[u], c, Rr);
+    };
+    auto consume = [&](int32_t t0, const V (&d)[U][NV], const float (&v)[U]) {
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int32_t t = t0 + u;
@@ -162,7 +239,7 @@ __device__ __forceinline__ void edge_block(const CsfDev& B, const FactorsDev& F,
               if (k >= 1) {
                 RO::hfma(acc[k - 1], acc[k], rowc[k]);
               } else {
-                const int64_t row = B.crd0[old];
+                const int64_t row = CRD(0, old);
                 if (old == head && !head_closed) {
                   head_closed = true;
                   if constexpr (EP) {
@@ -177,15 +254,15 @@ __device__ __forceinline__ void edge_block(const CsfDev& B, const FactorsDev& F,
               }
               RO::zero(acc[k]);
               int32_t x = old + 1;
-              int32_t e = B.pos[k + 1][x + 1];
+              int32_t e = POS(k, x + 1);
               if (e <= ckey) {
-                x = search_le(B.pos[k + 1], x + 1, hib[k], ckey);
-                e = B.pos[k + 1][x + 1];
+                x = SEARCH(k, x + 1, hib[k], ckey);
+                e = POS(k, x + 1);
               }
               cur[k] = x;
               cend[k] = e;
               if (k >= 1) {
-                RO::load(rowc[k], F.M[k], B.crd[k][x], c, Rr);
+                RO::load(rowc[k], F.M[k], CRD(k, x), c, Rr);
                 closing = x >= cend[k - 1];
                 ckey = x;
               }
@@ -194,11 +271,32 @@ __device__ __forceinline__ void edge_block(const CsfDev& B, const FactorsDev& F,
         }
         RO::axpy(acc[NL - 1], v[u], d[u]);
       }
+    };
+    V dA[U][NV], dB[U][NV];
+    float vA[U], vB[U];
+    int32_t lA, lB;
+    float wA, wB;
+    load_pairs(p0, lA, wA);
+    load_pairs(p0 + U, lB, wB);
+    issue(lA, wA, dA, vA);
+    for (int32_t t0 = p0; t0 < p1; t0 += 2 * U) {
+      // invariant: rows of batch t0 in dA; pairs of batch t0+U in (lB, wB)
+      if (t0 + U < p1) {
+        load_pairs(t0 + 2 * U, lA, wA);
+        issue(lB, wB, dB, vB);
+      }
+      consume(t0, dA, vA);
+      if (t0 + U >= p1) break;
+      if (t0 + 2 * U < p1) {
+        load_pairs(t0 + 3 * U, lB, wB);
+        issue(lA, wA, dA, vA);
+      }
+      consume(t0 + U, dB, vB);
     }
     // fold the open nodes into the open slice
 #pragma unroll
     for (int k = NL - 1; k >= 1; --k) RO::hfma(acc[k - 1], acc[k], rowc[k]);
-    if constexpr (!EP) RO::atomic_add(A, B.crd0[cur[0]], c, Rr, acc[0]);
+    if constexpr (!EP) RO::atomic_add(A, CRD(0, cur[0]), c, Rr, acc[0]);
   }
 
   if constexpr (EP) {
@@ -208,7 +306,6 @@ __device__ __forceinline__ void edge_block(const CsfDev& B, const FactorsDev& F,
     // 2g+1 = (open slice at run end, its partial).
     const int NGB = blockDim.x / LPR;
     const int E = 2 * NGB;
-    const int Rc = (R == 0) ? Rr : R;
     int32_t* keys = reinterpret_cast<int32_t*>(smem);
     float* ev = reinterpret_cast<float*>(smem + size_t(E) * sizeof(int32_t));
     V val[2][NV];
@@ -278,11 +375,12 @@ __device__ __forceinline__ void edge_block(const CsfDev& B, const FactorsDev& F,
       const int32_t k = keys[e];
       if (k == INT_MAX) continue;
       if (e > 0 && keys[e - 1] == k) continue;
-      const int64_t row = B.crd0[k];
+      const int64_t row = CRD(0, k);
       if (k == first_key || k == last_key) RO::atomic_add(A, row, c, Rr, val[q]);
       else RO::store(A, row, c, Rr, val[q]);
     }
   }
+  (void)head_row;
 }
 
 template <int N, int R, bool EP>
@@ -609,12 +707,9 @@ __global__ __launch_bounds__(512) void batch_kernel(const BatchTaskDev* __restri
 template <int R>
 int launch_batch_typed(const BatchLaunch& L, hipStream_t st) {
   const int lpr = Row<R>::LPR;
-  size_t lds = 0;
-  if (L.family == kEdgeParallel) {
-    const int E = 2 * (L.block / lpr);
-    lds = size_t(E) * sizeof(int32_t) + size_t(E) * L.R * sizeof(float);
-    if (lds > 65536) return fail(TENTACO_EUNSUPPORTED, "batch LDS %zu B", lds);
-  }
+  size_t lds = edge_stage_ints(kMaxOrder, L.NB) * sizeof(int32_t);
+  if (L.family == kEdgeParallel) lds += edge_ep_bytes(L.block, lpr, L.R);
+  if (lds > 65536) return fail(TENTACO_EUNSUPPORTED, "batch LDS %zu B", lds);
   {
     const int64_t work = std::max<int64_t>(L.total_fill, L.total_bnd);
     batch_prep_kernel<<<grid_for(work, 256, 16384), 256, 0, st>>>(
@@ -627,23 +722,22 @@ int launch_batch_typed(const BatchLaunch& L, hipStream_t st) {
     batch_kernel<R, true><<<grid, L.block, lds, st>>>(L.tasks, L.ntasks, L.item_off,
                                                        L.total_items, L.counter, L.R, L.group, L.NB);
   else
-    batch_kernel<R, false><<<grid, L.block, 0, st>>>(L.tasks, L.ntasks, L.item_off,
-                                                      L.total_items, L.counter, L.R, L.group, L.NB);
+    batch_kernel<R, false><<<grid, L.block, lds, st>>>(L.tasks, L.ntasks, L.item_off,
+                                                        L.total_items, L.counter, L.R, L.group, L.NB);
   TT_HIP_CHECK(hipGetLastError());
   return TENTACO_OK;
 }
 
 template <int R>
-int batch_occupancy_typed(int family, int block, int Rr, int* per_cu) {
+int batch_occupancy_typed(int family, int block, int Rr, int32_t NB, int* per_cu) {
   const int lpr = Row<R>::LPR;
-  size_t lds = 0;
   hipError_t e;
+  size_t lds = edge_stage_ints(kMaxOrder, NB) * sizeof(int32_t);
   if (family == kEdgeParallel) {
-    const int E = 2 * (block / lpr);
-    lds = size_t(E) * sizeof(int32_t) + size_t(E) * Rr * sizeof(float);
+    lds += edge_ep_bytes(block, lpr, Rr);
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, batch_kernel<R, true>, block, lds);
   } else {
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, batch_kernel<R, false>, block, 0);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, batch_kernel<R, false>, block, lds);
   }
   if (e != hipSuccess) return fail(TENTACO_EHIP, "occupancy query: %s", hipGetErrorString(e));
   return TENTACO_OK;
@@ -682,17 +776,19 @@ int launch_typed(const CsfDev& B, const FactorsDev& F, int Rr, float* A,
     TT_HIP_CHECK(hipGetLastError());
   }
   if (nblk == 0) return TENTACO_OK;
-  const int Rc = Rr;
+  const int lpr = Row<R>::LPR;
+  const size_t stage = edge_stage_ints(N, int32_t(NB)) * sizeof(int32_t);
   if (vr.family == kEdgeSerial) {
-    edge_kernel<N, R, false><<<int(nblk), vr.block, 0, st>>>(
+    if (stage > 65536)
+      return fail(TENTACO_EUNSUPPORTED, "edge-serial tile of %lld nonzeros needs %zu B of LDS",
+                  (long long)NB, stage);
+    edge_kernel<N, R, false><<<int(nblk), vr.block, stage, st>>>(
         B, F, A, Rr, brk, vr.group, int32_t(NB));
   } else {
-    const int lpr = Row<R>::LPR;
-    const int E = 2 * (vr.block / lpr);
-    const size_t lds = size_t(E) * sizeof(int32_t) + size_t(E) * Rc * sizeof(float);
+    const size_t lds = edge_ep_bytes(vr.block, lpr, Rr) + stage;
     if (lds > 65536)
       return fail(TENTACO_EUNSUPPORTED, "edge-parallel block %d at rank %d needs %zu B of LDS",
-                  vr.block, Rc, lds);
+                  vr.block, Rr, lds);
     edge_kernel<N, R, true><<<int(nblk), vr.block, lds, st>>>(
         B, F, A, Rr, brk, vr.group, int32_t(NB));
   }
@@ -731,11 +827,11 @@ int launch_batch(const BatchLaunch& L, hipStream_t st) {
   }
 }
 
-int batch_occupancy(int family, int block, int R, int* per_cu) {
+int batch_occupancy(int family, int block, int R, int32_t NB, int* per_cu) {
   switch (R) {
-    case 16: return batch_occupancy_typed<16>(family, block, R, per_cu);
-    case 32: return batch_occupancy_typed<32>(family, block, R, per_cu);
-    case 64: return batch_occupancy_typed<64>(family, block, R, per_cu);
+    case 16: return batch_occupancy_typed<16>(family, block, R, NB, per_cu);
+    case 32: return batch_occupancy_typed<32>(family, block, R, NB, per_cu);
+    case 64: return batch_occupancy_typed<64>(family, block, R, NB, per_cu);
     default: return fail(TENTACO_EUNSUPPORTED, "batched launch supports R in {16,32,64}");
   }
 }
