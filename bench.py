@@ -17,17 +17,137 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
 """
 
 import argparse
+import glob
 import json
 import os
+import shutil
+import signal
+import sqlite3
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 
+COPY_BYTES = 64 << 20  # calibration copy: 64 MiB read, 64 MiB written
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+# ---------------------------------------------------------------------------
+# hardware byte counters: two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE)
+# over the measured kernel, each scaled by what the same counter reads for a
+# plain copy of known size
+def profile_step(args):
+    """Child mode run under rocprofv3: only the measured kernel, W + K times."""
+    import tentaco_amd as tt
+    from tentaco_amd import metrics
+
+    n = args.warmup + args.steps
+    if args.profile_step == "copy":
+        src = tt.DeviceArray.empty((COPY_BYTES // 4,))
+        dst = tt.DeviceArray.empty((COPY_BYTES // 4,))
+        tt.flush_cache()
+        for _ in range(n):
+            tt.dev_copy(dst, src)
+        tt.synchronize()
+        return
+    B = metrics.workload_mttkrp3_uniform()
+    R = 32
+    facs = metrics.factors(B, R, 1001)
+    H = tt.DeviceCsf(B)
+    fd = [tt.DeviceArray.from_numpy(f) for f in facs]
+    A = tt.DeviceArray.empty((B.dims[0], R))
+    v = tt.variants().index(args.variant)
+    H.reserve(R, v)
+    for _ in range(n):
+        H.mttkrp(fd, A, R, v)
+    tt.synchronize()
+
+
+def _pmc_values(db_path, kernel, last_k):
+    """Per-dispatch counter sums of the last `last_k` dispatches whose kernel
+    name contains `kernel` (rocpd database written by rocprofv3)."""
+    con = sqlite3.connect(db_path)
+    try:
+        pc = [r[1] for r in con.execute("pragma table_info(pmc_events)")]
+        kc = [r[1] for r in con.execute("pragma table_info(kernels)")]
+        names = {}
+        for r in con.execute("select * from kernels"):
+            d = dict(zip(kc, r))
+            names[d["dispatch_id"]] = d["name"]
+        per = {}
+        for r in con.execute("select * from pmc_events"):
+            d = dict(zip(pc, r))
+            did = d["dispatch_id"]
+            if kernel in names.get(did, ""):
+                per[did] = per.get(did, 0.0) + float(d.get("value") or d.get("counter_value") or 0.0)
+    finally:
+        con.close()
+    vals = [per[k] for k in sorted(per)][-last_k:]
+    return vals
+
+
+def _pmc_pass(counter, step_args, kernel, last_k, timeout=120):
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        raise RuntimeError("rocprofv3 not found")
+    out = tempfile.mkdtemp(prefix="tt_pmc_")
+    cmd = [exe, "--pmc", counter, "-d", out, "-o", "run", "--",
+           sys.executable, os.path.abspath(__file__)] + step_args
+    with open(os.path.join(out, "log.txt"), "w") as lf:
+        p = subprocess.Popen(cmd, stdout=lf, stderr=subprocess.STDOUT, start_new_session=True)
+        try:
+            rc = p.wait(timeout=timeout)
+        except subprocess.TimeoutExpired:
+            os.killpg(p.pid, signal.SIGKILL)
+            p.wait()
+            raise RuntimeError(f"rocprofv3 --pmc {counter} timed out")
+    dbs = glob.glob(os.path.join(out, "**", "*.db"), recursive=True)
+    if not dbs:
+        raise RuntimeError(f"rocprofv3 --pmc {counter} wrote no database (exit {rc})")
+    vals = _pmc_values(dbs[0], kernel, last_k)
+    shutil.rmtree(out, ignore_errors=True)
+    if len(vals) < last_k:
+        raise RuntimeError(f"{counter}: {len(vals)} dispatches of {kernel}, expected {last_k}")
+    vals.sort()
+    return vals[len(vals) // 2]  # median KiB per dispatch
+
+
+def measure_traffic(variant, kernel, steps=10, warmup=3):
+    """HBM-side bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, in KiB
+    per dispatch, each scaled by copy_bytes / counter(copy of copy_bytes))."""
+    mt = ["--profile-step", "mttkrp3", "--variant", variant,
+          "--steps", str(steps), "--warmup", str(warmup)]
+    cp = ["--profile-step", "copy", "--steps", str(steps), "--warmup", str(warmup)]
+    f_k = _pmc_pass("FETCH_SIZE", mt, kernel, steps)
+    w_k = _pmc_pass("WRITE_SIZE", mt, kernel, steps)
+    f_c = _pmc_pass("FETCH_SIZE", cp, "copy_kernel", steps)
+    w_c = _pmc_pass("WRITE_SIZE", cp, "copy_kernel", steps)
+    ff = COPY_BYTES / (f_c * 1024.0)
+    wf = COPY_BYTES / (w_c * 1024.0)
+    fetch = f_k * 1024.0 * ff
+    write = w_k * 1024.0 * wf
+    return {
+        "traffic": int(round(fetch + write)),
+        "fetch_bytes": int(round(fetch)),
+        "write_bytes": int(round(write)),
+        "fetch_kib_raw": f_k,
+        "write_kib_raw": w_k,
+        "copy_bytes": COPY_BYTES,
+        "copy_fetch_kib": f_c,
+        "copy_write_kib": w_c,
+        "fetch_scale": round(ff, 4),
+        "write_scale": round(wf, 4),
+        "kernel": kernel,
+        "variant": variant,
+        "dispatches_per_pass": steps,
+    }
 
 
 def time_steps(step, K):
@@ -79,12 +199,20 @@ def run_mttkrp_workload(tt, B, R, seed, K, W, reps=5, check=True, fp32_check=Tru
     for _ in range(W):
         step()
     total_ms, wall_ms = time_steps(step, K)
+    # the main kernel alone: events on its launch stream around each launch
+    tt.ktimer_start()
+    for _ in range(K):
+        step()
+    k_total, k_n, k_name = tt.ktimer_stop()
     warm = per_step_ms(step, K, cold=False)
     cold = per_step_ms(step, K, cold=True)
     res = {
         "variant": names[best],
         "ms_per_step": total_ms / K,
         "wall_ms_per_step": wall_ms / K,
+        "kernel": k_name,
+        "kernel_ms_avg": k_total / max(k_n, 1),
+        "kernel_launches_timed": k_n,
         "warm_step_ms_avg": float(np.mean(warm)),
         "cold_step_ms_min": float(np.min(cold)),
         "cold_step_ms_median": float(np.median(cold)),
@@ -202,10 +330,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--quick", action="store_true", help="skip the 100M-nnz and TTV extra lines")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 FETCH_SIZE/WRITE_SIZE passes")
+    ap.add_argument("--profile-step", choices=["mttkrp3", "copy"], default=None,
+                    help=argparse.SUPPRESS)
+    ap.add_argument("--variant", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.profile_step:
+        profile_step(args)
+        return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU "
-              "sharding of this path is a design described in DESIGN.md, not a measured run.")
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): this build has no "
+              "multi-GPU path; the slice-range sharding of MTTKRP is a design, not a run.")
         sys.exit(2)
     K, W = args.steps, args.warmup
 
@@ -229,8 +365,28 @@ def main():
         sys.exit(1)
     ms = res["ms_per_step"]
     value = balg / (ms * 1e-3) / 1e9
-    kern_ms = res["warm_step_ms_avg"]
+    # A slice-row step on a dense root launches the main kernel alone (no
+    # partition or zero pre-pass), so the events over the K-step region give
+    # that kernel's average launch duration directly; otherwise use the
+    # per-launch event pairs around the main kernel (they add a few
+    # microseconds of event overhead per launch).
+    n0 = B.nodes(0)
+    single = res["variant"].startswith("sr_") and n0 == B.dims[0] and \
+        bool(np.array_equal(B.crd[0], np.arange(n0, dtype=B.crd[0].dtype)))
+    kern_ms = ms if single else res["kernel_ms_avg"]
+    res["kernel_time_source"] = "events over the timed region" if single else \
+        "event pair per launch"
     achieved = balg / (kern_ms * 1e-3) / 1e9
+    traffic = None
+    if not args.no_traffic:
+        try:
+            log(f"rocprofv3 FETCH_SIZE / WRITE_SIZE passes on {res['kernel']} and a "
+                f"{COPY_BYTES >> 20} MiB copy")
+            traffic = measure_traffic(res["variant"], res["kernel"])
+            log(f"traffic {traffic['traffic']} B per launch (algorithmic {balg} B)")
+        except Exception as e:  # the counters are a report, not the measurement
+            log(f"traffic not measured: {e}")
+            traffic = None
 
     cpu = None
     if not args.no_cpu:
@@ -258,6 +414,7 @@ def main():
         "c1": {k: v for k, v in res.items() if k != "sweep"},
         "c1_sweep_ms": res["sweep"],
         "cold_gbps": round(balg / (res["cold_step_ms_median"] * 1e-3) / 1e9, 2),
+        "traffic_detail": traffic,
     }
     if not args.quick:
         log("generating the 500k^3 / 100M-nonzero Zipf tensor")
@@ -335,7 +492,7 @@ def main():
             "peak": metrics.HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(achieved / metrics.HBM_PEAK_GBPS, 5),
-            "traffic": None,
+            "traffic": traffic["traffic"] if traffic else None,
         },
         "cpu_baseline": cpu,
         "extra": extra,

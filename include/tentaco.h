@@ -275,6 +275,21 @@ int tentaco_flush_cache(void* stream);
 int tentaco_timer_start(void* stream);
 int tentaco_timer_stop(void* stream, float* ms);
 
+/* Kernel timer: while on, every call records a hipEvent pair on its launch
+ * stream around its main kernel only (slice_kernel, edge_kernel, ttv_kernel,
+ * batch_kernel or copy_kernel; not the partition/zero pre-pass), at most 4096
+ * launches; never inside a stream capture. stop() turns it off and returns
+ * the summed duration, the launch count and the last kernel's name. No
+ * reference counterpart (GPUTimer, gpu_library.h:949-988, brackets the whole
+ * region). */
+int tentaco_ktimer_start(void);
+int tentaco_ktimer_stop(double* total_ms, int* launches, char* kernel, int kernel_len);
+
+/* Plain device-to-device copy (16-byte aligned, non-overlapping) through a
+ * streaming copy kernel on `stream`: the known-size reference workload for
+ * calibrating hardware byte counters, and a staging helper. */
+int tentaco_dev_copy(void* dst, const void* src, size_t bytes, void* stream);
+
 /* ---- host-side CSF construction ---------------------------------------- */
 /* An owned host CSF (dense root, int32 indices, FP32 values) with a
  * taco_tensor_t view. Replaces the reference's splatt load path

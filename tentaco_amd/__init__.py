@@ -124,6 +124,9 @@ def lib() -> ctypes.CDLL:
         "tentaco_timer_start": (c_int, [c_void_p]),
         "tentaco_flush_cache": (c_int, [c_void_p]),
         "tentaco_timer_stop": (c_int, [c_void_p, POINTER(c_float)]),
+        "tentaco_ktimer_start": (c_int, []),
+        "tentaco_ktimer_stop": (c_int, [POINTER(ctypes.c_double), POINTER(c_int), ctypes.c_char_p, c_int]),
+        "tentaco_dev_copy": (c_int, [c_void_p, c_void_p, ctypes.c_size_t, c_void_p]),
         "tentaco_coo_to_csf": (c_int, [c_int, POINTER(c_int32), c_int64, POINTER(c_int32), POINTER(c_float), POINTER(c_void_p)]),
         "tentaco_synth_csf": (c_int, [POINTER(SynthParams), POINTER(c_void_p)]),
         "tentaco_host_csf_view": (P, [c_void_p]),
@@ -532,3 +535,24 @@ def timer_stop(stream=None) -> float:
     ms = c_float()
     _check(lib().tentaco_timer_stop(stream, ctypes.byref(ms)))
     return float(ms.value)
+
+
+def ktimer_start():
+    """Start timing the main kernel of every following call (events on the
+    launch stream around that kernel only)."""
+    _check(lib().tentaco_ktimer_start())
+
+
+def ktimer_stop():
+    """Stop the kernel timer; returns (total_ms, launches, kernel_name)."""
+    total = ctypes.c_double()
+    n = c_int()
+    name = ctypes.create_string_buffer(64)
+    _check(lib().tentaco_ktimer_stop(ctypes.byref(total), ctypes.byref(n), name, 64))
+    return float(total.value), int(n.value), name.value.decode()
+
+
+def dev_copy(dst, src, nbytes=None, stream=None):
+    """Device-to-device copy of DeviceArrays through the streaming copy kernel."""
+    nbytes = min(dst.nbytes, src.nbytes) if nbytes is None else int(nbytes)
+    _check(lib().tentaco_dev_copy(dst.ptr, src.ptr, nbytes, stream))

@@ -159,6 +159,8 @@ int tentaco_batch_set_operands(tentaco_batch* b, const float* const* factors_dev
       return fail(TENTACO_EINVAL, "task %d output has %d rows, root coordinates reach %d", t,
                   out_rows[t], h->max_crd[0]);
     if (!outs_dev[t]) return fail(TENTACO_EINVAL, "task %d output is NULL", t);
+    if (reinterpret_cast<uintptr_t>(outs_dev[t]) & 15)
+      return fail(TENTACO_EINVAL, "task %d output is not 16-byte aligned", t);
     BatchTaskDev& T = b->host_tasks[t];
     T.B = h->view();
     for (int m = 1; m < h->order; ++m) {

@@ -101,6 +101,11 @@ int batch_occupancy(int family, int block, int R, int32_t NB, int* per_cu);
 // safe inside captured graphs).
 int launch_fill(float* p, int64_t n, float value, hipStream_t stream);
 
+// Event timing of the main kernel of a call (no-op unless the kernel timer
+// is on): begin before the launch, end after it, on the launch stream.
+void ktimer_begin(hipStream_t stream);
+void ktimer_end(hipStream_t stream, const char* kernel);
+
 // Error plumbing: record a message for tentaco_last_error() and return code.
 int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 void clear_error();

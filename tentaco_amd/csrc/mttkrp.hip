@@ -656,9 +656,31 @@ __global__ __launch_bounds__(256) void batch_prep_kernel(const BatchTaskDev* __r
   const int64_t tid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t nthr = int64_t(gridDim.x) * blockDim.x;
   if (tid == 0) *counter = 0;
-  for (int64_t i = tid; i < total_fill; i += nthr) {
-    const int32_t t = upper_index(fill_off, ntasks, i);
-    tasks[t].A[i - fill_off[t]] = 0.f;
+  // Zero the outputs as float4 quads (every output holds rows x R floats
+  // with R a multiple of 4 and starts 16-byte aligned, which the host
+  // checks). Each workgroup takes one contiguous range of quads, finds the
+  // task of its first quad with one search and walks forward across task
+  // boundaries (empty tasks included).
+  const int64_t total_q = total_fill >> 2;
+  const int64_t per = (total_q + gridDim.x - 1) / gridDim.x;
+  const int64_t q0 = int64_t(blockIdx.x) * per;
+  const int64_t q1 = q0 + per < total_q ? q0 + per : total_q;
+  if (q0 < q1) {
+    int32_t t = upper_index(fill_off, ntasks, q0 << 2);
+    int64_t base = fill_off[t] >> 2;
+    int64_t next = t + 1 < ntasks ? fill_off[t + 1] >> 2 : total_q;
+    float4* A4 = reinterpret_cast<float4*>(tasks[t].A);
+    for (int64_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+      if (q >= next) {
+        do {
+          ++t;
+          next = t + 1 < ntasks ? fill_off[t + 1] >> 2 : total_q;
+        } while (q >= next);
+        base = fill_off[t] >> 2;
+        A4 = reinterpret_cast<float4*>(tasks[t].A);
+      }
+      A4[q - base] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
   }
   for (int64_t i = tid; i < total_bnd; i += nthr) {
     const int32_t t = upper_index(bnd_off, ntasks, i);
@@ -718,6 +740,7 @@ int launch_batch_typed(const BatchLaunch& L, hipStream_t st) {
   }
   if (L.total_items == 0) return TENTACO_OK;
   const int grid = std::max(1, std::min<int>(L.grid, L.total_items));
+  ktimer_begin(st);
   if (L.family == kEdgeParallel)
     batch_kernel<R, true><<<grid, L.block, lds, st>>>(L.tasks, L.ntasks, L.item_off,
                                                        L.total_items, L.counter, L.R, L.group, L.NB);
@@ -725,6 +748,7 @@ int launch_batch_typed(const BatchLaunch& L, hipStream_t st) {
     batch_kernel<R, false><<<grid, L.block, lds, st>>>(L.tasks, L.ntasks, L.item_off,
                                                         L.total_items, L.counter, L.R, L.group, L.NB);
   TT_HIP_CHECK(hipGetLastError());
+  ktimer_end(st, "batch_kernel");
   return TENTACO_OK;
 }
 
@@ -761,8 +785,10 @@ int launch_typed(const CsfDev& B, const FactorsDev& F, int Rr, float* A,
       TT_HIP_CHECK(hipGetLastError());
     }
     if (B.nodes[0] > 0) {
+      ktimer_begin(st);
       slice_kernel<N, R><<<B.nodes[0], vr.block, 0, st>>>(B, F, A, Rr);
       TT_HIP_CHECK(hipGetLastError());
+      ktimer_end(st, "slice_kernel");
     }
     return TENTACO_OK;
   }
@@ -782,6 +808,7 @@ int launch_typed(const CsfDev& B, const FactorsDev& F, int Rr, float* A,
     if (stage > 65536)
       return fail(TENTACO_EUNSUPPORTED, "edge-serial tile of %lld nonzeros needs %zu B of LDS",
                   (long long)NB, stage);
+    ktimer_begin(st);
     edge_kernel<N, R, false><<<int(nblk), vr.block, stage, st>>>(
         B, F, A, Rr, brk, vr.group, int32_t(NB));
   } else {
@@ -789,10 +816,12 @@ int launch_typed(const CsfDev& B, const FactorsDev& F, int Rr, float* A,
     if (lds > 65536)
       return fail(TENTACO_EUNSUPPORTED, "edge-parallel block %d at rank %d needs %zu B of LDS",
                   vr.block, Rr, lds);
+    ktimer_begin(st);
     edge_kernel<N, R, true><<<int(nblk), vr.block, lds, st>>>(
         B, F, A, Rr, brk, vr.group, int32_t(NB));
   }
   TT_HIP_CHECK(hipGetLastError());
+  ktimer_end(st, "edge_kernel");
   return TENTACO_OK;
 }
 

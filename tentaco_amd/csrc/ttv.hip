@@ -174,8 +174,10 @@ int launch_ttv(const CsfDev& B, const float* c, float* y, void* ws,
   ttv_prep_kernel<<<int(g), 256, 0, st>>>(pos, y, nf, B.nnz, brk, int32_t(nblk));
   TT_HIP_CHECK(hipGetLastError());
   if (nblk == 0) return TENTACO_OK;
+  ktimer_begin(st);
   ttv_kernel<<<int(nblk), kTtvBlock, 0, st>>>(pos, crd, B.vals, c, y, B.nnz, brk);
   TT_HIP_CHECK(hipGetLastError());
+  ktimer_end(st, "ttv_kernel");
   return TENTACO_OK;
 }
 
