@@ -204,18 +204,105 @@ int tentaco_csf_reserve(tentaco_csf* h, int R, int variant);
 int tentaco_mttkrp_dev(tentaco_csf* h, int R, const float* const* factors_dev,
                        float* A_dev, int32_t a_rows, int variant,
                        void* stream);
+/* Sweep (the measurement of codegen.py + the *_tune.sh loops of the
+ * reference, ebsr_32_tune.sh:1-16): times every variant on this handle, rank
+ * and device factors with hipEvents on `stream`. Each variant gets a warm-up
+ * and a one-call probe (the timed warm-up itself when it exceeds 2 ms);
+ * every variant whose probe is within `cut` x the
+ * fastest probe (all of them when cut <= 0) is then timed `reps` times, each
+ * repetition spanning about 100 us of back-to-back calls (at most 32), and
+ * its median per-call time kept. times_ms (tentaco_num_variants() entries)
+ * receives per-call ms: the median, the probe for variants past the cut, -1
+ * for a variant this rank cannot run. Reserves the largest workspace first;
+ * changes nothing else on the handle; not for graph capture. */
+int tentaco_csf_sweep(tentaco_csf* h, int R, const float* const* factors_dev,
+                      float* A_dev, int32_t a_rows, int reps, float cut,
+                      void* stream, float* times_ms);
 /* Autotuner (replaces codegen.py + the *_tune.sh sweeps of the reference):
- * times every variant reps times on this handle, rank and device factors
- * (hipEvents on `stream`, median per variant), records the fastest in the
+ * tentaco_csf_sweep with cut = 4, then records the fastest variant in the
  * handle so TENTACO_VARIANT_AUTO uses it, and returns it in *best. times_ms
- * (optional, tentaco_num_variants() entries) receives each median, -1 for a
- * variant this rank cannot run. Allocates workspace; not for graph capture. */
+ * (optional) receives the sweep's times. Not for graph capture. */
 int tentaco_csf_autotune(tentaco_csf* h, int R, const float* const* factors_dev,
                          float* A_dev, int32_t a_rows, int reps, void* stream,
                          int* best, float* times_ms);
 /* Record a variant as this handle's choice for rank R (e.g. from a stored
  * sweep or the learned selector). */
 int tentaco_csf_set_tuned(tentaco_csf* h, int R, int variant);
+
+/* How a handle's AUTO choice for rank R was made. */
+#define TENTACO_TUNED_NONE 0      /* no entry: AUTO uses the selector or heuristic */
+#define TENTACO_TUNED_SWEEP 1     /* measured by the sweep (autotune) */
+#define TENTACO_TUNED_SELECTOR 2  /* the learned selector's pick */
+#define TENTACO_TUNED_SET 3       /* set by tentaco_csf_set_tuned */
+/* The recorded choice for rank R: *variant = -1 and *source = NONE if none. */
+int tentaco_csf_tuned(const tentaco_csf* h, int R, int* variant, int* source);
+
+/* ---- learned variant selector ------------------------------------------ */
+/* No reference counterpart: it replaces running the exhaustive sweep
+ * (codegen.py:44-51 + ebsr_32_tune.sh / ebpr_*_tune.sh) for every tensor with
+ * a prediction from the tensor's structure, trained on the sweep's own
+ * timings; the sweep stays as the fallback and as the judge of its picks. */
+#define TENTACO_NUM_FEATURES 24
+#define TENTACO_MLP_HIDDEN 64
+/* The 24 features of an uploaded CSF at rank R:
+ *   [0]      ln nnz
+ *   [1..5]   ln dims[m], m = 0..4 (0 past the order)
+ *   [6..10]  ln node count of level m, m = 0..4 (0 past the order)
+ *   [11..22] for levels m = 1..4, three each: ln(1 + mean), ln(1 + variance),
+ *            ln(1 + max) of the children per level m-1 node (0 past the order)
+ *   [23]     ln R */
+int tentaco_csf_features(const tentaco_csf* h, int R, float* features);
+
+/* An FP32 MLP 24 -> 64 -> 64 -> n_out (ReLU) resident on the current device
+ * (n_out <= 64; n_out = tentaco_num_variants() for selection). Outputs are
+ * predicted ln(ms) per variant. Parameters (tentaco_mlp_num_params floats)
+ * are W1[64][24] b1[64] W2[64][64] b2[64] W3[n_out][64] b3[n_out], row-major
+ * [out][in]; create() draws He-normal weights from seed, zero biases.
+ * Inputs are standardised as (x - x_mean) / x_std and outputs mapped back as
+ * y * y_std + y_mean (set_scaling; identity until set). */
+typedef struct tentaco_mlp tentaco_mlp;
+int64_t tentaco_mlp_num_params(int n_out);
+int tentaco_mlp_create(int n_out, uint64_t seed, tentaco_mlp** out);
+int tentaco_mlp_free(tentaco_mlp* m);
+int tentaco_mlp_set_scaling(tentaco_mlp* m, const float* x_mean, const float* x_std,
+                            float y_mean, float y_std);
+int tentaco_mlp_get_scaling(const tentaco_mlp* m, float* x_mean, float* x_std,
+                            float* y_mean, float* y_std);
+int tentaco_mlp_get_params(const tentaco_mlp* m, float* host, int64_t n);
+int tentaco_mlp_set_params(tentaco_mlp* m, const float* host, int64_t n);
+/* Gradients of the last loss_grad / training step (host copy). */
+int tentaco_mlp_get_grads(const tentaco_mlp* m, float* host, int64_t n);
+/* Inference kernel on device rows: x_dev n x 24 raw features -> y_dev
+ * n x n_out predicted ln(ms), on `stream`. */
+int tentaco_mlp_forward(tentaco_mlp* m, const float* x_dev, int n, float* y_dev,
+                        void* stream);
+/* Forward + backward kernels on host arrays x (n x 24), t (n x n_out target
+ * ln(ms)) and mask (n x n_out, 1 = use): loss = sum mask * (y_s - t_s)^2 /
+ * sum mask over the standardised prediction y_s and target t_s; the
+ * gradients stay in the model (get_grads). Synchronous. */
+int tentaco_mlp_loss_grad(tentaco_mlp* m, const float* x, const float* t,
+                          const float* mask, int n, float* loss);
+/* One Adam step (beta1, beta2, eps) with the stored gradients. */
+int tentaco_mlp_adam(tentaco_mlp* m, float lr, float beta1, float beta2, float eps);
+/* Training loop on the device: the set (host arrays as in loss_grad) is
+ * uploaded once; each epoch shuffles it (seeded Fisher-Yates) into
+ * minibatches of `batch`, each a gather, forward, backward and Adam kernel
+ * (beta 0.9 / 0.999, eps 1e-8, step size lr with cosine decay over the run).
+ * *final_loss = mean loss over the last epoch. Deterministic for a seed. */
+int tentaco_mlp_train(tentaco_mlp* m, const float* x, const float* t, const float* mask,
+                      int n, int epochs, int batch, float lr, uint64_t seed,
+                      float* final_loss);
+/* Pick: the variant with the smallest predicted time among those that fit an
+ * order-`order` tensor at rank R; *margin = predicted runner-up / best - 1;
+ * pred_ms (optional, n_out entries) = exp of the predictions. */
+int tentaco_mlp_select(tentaco_mlp* m, int order, int R, const float* features,
+                       int* variant, float* margin, float* pred_ms);
+/* Attach a selector (not owned; NULL detaches) to a handle: AUTO on a rank
+ * with no recorded choice then takes the selector's pick when its margin is
+ * at least min_margin (0.05 = 5%) and records it (source SELECTOR); below
+ * that it runs tentaco_csf_autotune (source SWEEP). Under stream capture the
+ * structural heuristic is used instead, since both would synchronise. */
+int tentaco_csf_set_selector(tentaco_csf* h, tentaco_mlp* m, float min_margin);
 
 /* TTV on device buffers: c_dev has the last mode's extent, A_dev has one
  * float per level N-2 node (nodes[N-2]). */

@@ -24,6 +24,8 @@ LIB_PATH = os.path.join(_HERE, "libtentaco.so")
 
 VARIANT_AUTO = -1
 VARIANT_DEFAULT = -2
+NUM_FEATURES = 24
+TUNED_NONE, TUNED_SWEEP, TUNED_SELECTOR, TUNED_SET = 0, 1, 2, 3
 
 OK, EINVAL, EHIP, ENOMEM, EUNSUPPORTED, ENOGPU = 0, 1, 2, 3, 4, 5
 
@@ -107,6 +109,28 @@ def lib() -> ctypes.CDLL:
         "tentaco_csf_autotune": (c_int, [c_void_p, c_int, POINTER(c_void_p), c_void_p, c_int32, c_int,
                                          c_void_p, POINTER(c_int), POINTER(c_float)]),
         "tentaco_csf_set_tuned": (c_int, [c_void_p, c_int, c_int]),
+        "tentaco_csf_sweep": (c_int, [c_void_p, c_int, POINTER(c_void_p), c_void_p, c_int32, c_int,
+                                      c_float, c_void_p, POINTER(c_float)]),
+        "tentaco_csf_tuned": (c_int, [c_void_p, c_int, POINTER(c_int), POINTER(c_int)]),
+        "tentaco_csf_features": (c_int, [c_void_p, c_int, POINTER(c_float)]),
+        "tentaco_csf_set_selector": (c_int, [c_void_p, c_void_p, c_float]),
+        "tentaco_mlp_num_params": (c_int64, [c_int]),
+        "tentaco_mlp_create": (c_int, [c_int, ctypes.c_uint64, POINTER(c_void_p)]),
+        "tentaco_mlp_free": (c_int, [c_void_p]),
+        "tentaco_mlp_set_scaling": (c_int, [c_void_p, POINTER(c_float), POINTER(c_float), c_float, c_float]),
+        "tentaco_mlp_get_scaling": (c_int, [c_void_p, POINTER(c_float), POINTER(c_float), POINTER(c_float),
+                                            POINTER(c_float)]),
+        "tentaco_mlp_get_params": (c_int, [c_void_p, POINTER(c_float), c_int64]),
+        "tentaco_mlp_set_params": (c_int, [c_void_p, POINTER(c_float), c_int64]),
+        "tentaco_mlp_get_grads": (c_int, [c_void_p, POINTER(c_float), c_int64]),
+        "tentaco_mlp_forward": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+        "tentaco_mlp_loss_grad": (c_int, [c_void_p, POINTER(c_float), POINTER(c_float), POINTER(c_float),
+                                          c_int, POINTER(c_float)]),
+        "tentaco_mlp_adam": (c_int, [c_void_p, c_float, c_float, c_float, c_float]),
+        "tentaco_mlp_train": (c_int, [c_void_p, POINTER(c_float), POINTER(c_float), POINTER(c_float), c_int,
+                                      c_int, c_int, c_float, ctypes.c_uint64, POINTER(c_float)]),
+        "tentaco_mlp_select": (c_int, [c_void_p, c_int, c_int, POINTER(c_float), POINTER(c_int),
+                                       POINTER(c_float), POINTER(c_float)]),
         "tentaco_batch_create": (c_int, [POINTER(c_void_p), c_int, c_int, POINTER(c_void_p)]),
         "tentaco_batch_set_operands": (c_int, [c_void_p, POINTER(c_void_p), POINTER(c_void_p), POINTER(c_int32)]),
         "tentaco_batch_configure": (c_int, [c_void_p, c_int, c_int, c_int]),
@@ -443,6 +467,36 @@ class DeviceCsf:
     def set_tuned(self, R: int, variant: int):
         _check(lib().tentaco_csf_set_tuned(self.h, R, variant))
 
+    def sweep(self, factors: Sequence[DeviceArray], A: DeviceArray, R: int, reps: int = 3,
+              cut: float = 4.0, stream=None, a_rows: Optional[int] = None) -> List[float]:
+        """Per-call ms of every variant (probe, then the median of `reps`
+        repetitions within `cut` x the fastest probe); -1 where unsupported.
+        Leaves the handle's recorded choices unchanged."""
+        arr = (c_void_p * len(factors))(*[f.ptr for f in factors])
+        rows = self.dims[0] if a_rows is None else a_rows
+        n = lib().tentaco_num_variants()
+        times = (c_float * n)()
+        _check(lib().tentaco_csf_sweep(self.h, R, arr, A.ptr, rows, reps, cut, stream, times))
+        return [float(t) for t in times]
+
+    def tuned(self, R: int):
+        """(variant, source) recorded for rank R; source is one of TUNED_*."""
+        v, src = c_int(), c_int()
+        _check(lib().tentaco_csf_tuned(self.h, R, ctypes.byref(v), ctypes.byref(src)))
+        return v.value, src.value
+
+    def features(self, R: int) -> np.ndarray:
+        """The 24 selector features of this tensor at rank R."""
+        f = np.zeros(NUM_FEATURES, np.float32)
+        _check(lib().tentaco_csf_features(self.h, R, f.ctypes.data_as(POINTER(c_float))))
+        return f
+
+    def set_selector(self, selector: Optional["Selector"], min_margin: float = 0.05):
+        """AUTO on an unrecorded rank uses the selector's pick, or the sweep
+        when the predicted margin is under min_margin."""
+        self._selector = selector  # keep it alive while attached
+        _check(lib().tentaco_csf_set_selector(self.h, selector.h if selector else None, min_margin))
+
     def ttv(self, c: DeviceArray, y: DeviceArray, stream=None):
         _check(lib().tentaco_ttv_dev(self.h, c.ptr, y.ptr, stream))
 
@@ -556,3 +610,110 @@ def dev_copy(dst, src, nbytes=None, stream=None):
     """Device-to-device copy of DeviceArrays through the streaming copy kernel."""
     nbytes = min(dst.nbytes, src.nbytes) if nbytes is None else int(nbytes)
     _check(lib().tentaco_dev_copy(dst.ptr, src.ptr, nbytes, stream))
+
+
+def _fp(a: np.ndarray):
+    return a.ctypes.data_as(POINTER(c_float))
+
+
+class Selector:
+    """The learned variant selector (FP32 MLP 24 -> 64 -> 64 -> K on the
+    device). Its forward, backward and Adam steps are the library's HIP
+    kernels; this class only moves host arrays in and out."""
+
+    def __init__(self, n_out: Optional[int] = None, seed: int = 0):
+        self.n_out = int(n_out or lib().tentaco_num_variants())
+        self.h = c_void_p()
+        _check(lib().tentaco_mlp_create(self.n_out, seed, ctypes.byref(self.h)))
+        self.n_params = int(lib().tentaco_mlp_num_params(self.n_out))
+
+    def set_scaling(self, x_mean, x_std, y_mean: float, y_std: float):
+        xm = np.ascontiguousarray(x_mean, np.float32)
+        xs = np.ascontiguousarray(x_std, np.float32)
+        _check(lib().tentaco_mlp_set_scaling(self.h, _fp(xm), _fp(xs), float(y_mean), float(y_std)))
+
+    def scaling(self):
+        xm = np.zeros(NUM_FEATURES, np.float32)
+        xs = np.zeros(NUM_FEATURES, np.float32)
+        ym, ys = c_float(), c_float()
+        _check(lib().tentaco_mlp_get_scaling(self.h, _fp(xm), _fp(xs), ctypes.byref(ym), ctypes.byref(ys)))
+        return xm, xs, float(ym.value), float(ys.value)
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.n_params, np.float32)
+        _check(lib().tentaco_mlp_get_params(self.h, _fp(p), self.n_params))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        _check(lib().tentaco_mlp_set_params(self.h, _fp(p), int(p.size)))
+
+    def grads(self) -> np.ndarray:
+        g = np.zeros(self.n_params, np.float32)
+        _check(lib().tentaco_mlp_get_grads(self.h, _fp(g), self.n_params))
+        return g
+
+    def forward(self, x) -> np.ndarray:
+        """Predicted ln(ms) per output for rows of raw features (inference kernel)."""
+        x = np.ascontiguousarray(np.atleast_2d(x), np.float32)
+        n = x.shape[0]
+        xd = DeviceArray.from_numpy(x)
+        yd = DeviceArray.empty((n, self.n_out))
+        _check(lib().tentaco_mlp_forward(self.h, xd.ptr, n, yd.ptr, None))
+        return yd.numpy()
+
+    def loss_grad(self, x, t, mask) -> float:
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        m = np.ascontiguousarray(mask, np.float32)
+        loss = c_float()
+        _check(lib().tentaco_mlp_loss_grad(self.h, _fp(x), _fp(t), _fp(m), x.shape[0], ctypes.byref(loss)))
+        return float(loss.value)
+
+    def adam(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        _check(lib().tentaco_mlp_adam(self.h, lr, beta1, beta2, eps))
+
+    def train(self, x, t, mask, epochs=300, batch=128, lr=3e-3, seed=0) -> float:
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        m = np.ascontiguousarray(mask, np.float32)
+        loss = c_float()
+        _check(lib().tentaco_mlp_train(self.h, _fp(x), _fp(t), _fp(m), x.shape[0], epochs, batch, lr,
+                                       seed, ctypes.byref(loss)))
+        return float(loss.value)
+
+    def select(self, order: int, R: int, features):
+        """(variant, margin, predicted ms per variant) for one tensor."""
+        f = np.ascontiguousarray(features, np.float32)
+        v, margin = c_int(), c_float()
+        pred = np.zeros(self.n_out, np.float32)
+        _check(lib().tentaco_mlp_select(self.h, order, R, _fp(f), ctypes.byref(v), ctypes.byref(margin),
+                                        _fp(pred)))
+        return v.value, float(margin.value), pred
+
+    def save(self, path: str, **meta):
+        xm, xs, ym, ys = self.scaling()
+        np.savez(path, params=self.params(), x_mean=xm, x_std=xs, y_mean=np.float32(ym),
+                 y_std=np.float32(ys), n_out=np.int32(self.n_out),
+                 variants=np.array(variants()), **meta)
+
+    @classmethod
+    def load(cls, path: str) -> "Selector":
+        d = np.load(path, allow_pickle=False)
+        if list(d["variants"]) != variants():
+            raise TentacoError(EINVAL, f"{path}: trained for a different variant table")
+        s = cls(int(d["n_out"]))
+        s.set_params(d["params"])
+        s.set_scaling(d["x_mean"], d["x_std"], float(d["y_mean"]), float(d["y_std"]))
+        return s
+
+    def close(self):
+        if self.h:
+            lib().tentaco_mlp_free(self.h)
+            self.h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

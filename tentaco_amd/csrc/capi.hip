@@ -3,6 +3,7 @@
 // reference's symbol names. Kernels live in mttkrp.hip / ttv.hip.
 
 #include <algorithm>
+#include <cmath>
 #include <atomic>
 #include <cstdarg>
 #include <cstdio>
@@ -84,9 +85,21 @@ int validate_csf(const taco_tensor_t* B, int order, tentaco_csf* h) {
     const int32_t* pos = B->indices[m][0];
     const int32_t* crd = B->indices[m][1];
     if (pos[0] != 0) return fail(TENTACO_EINVAL, "level %d pos[0] = %d", m, pos[0]);
-    for (int64_t x = 0; x < parents; ++x)
+    double sum = 0.0, sum2 = 0.0, cmax = 0.0;
+    for (int64_t x = 0; x < parents; ++x) {
       if (pos[x + 1] < pos[x])
         return fail(TENTACO_EINVAL, "level %d pos decreases at %lld", m, (long long)x);
+      const double c = double(pos[x + 1] - pos[x]);
+      sum += c;
+      sum2 += c * c;
+      cmax = c > cmax ? c : cmax;
+    }
+    if (parents > 0) {
+      const double mean = sum / double(parents);
+      h->stats.child_mean[m] = mean;
+      h->stats.child_var[m] = std::max(0.0, sum2 / double(parents) - mean * mean);
+      h->stats.child_max[m] = cmax;
+    }
     const int64_t nodes = pos[parents];
     if (nodes > kMaxNnz)
       return fail(TENTACO_EUNSUPPORTED, "level %d has %lld nodes (int32 index limit)", m,
@@ -394,6 +407,38 @@ int tentaco_csf_reserve(tentaco_csf* h, int R, int variant) {
   return reserve_ws(h, need);
 }
 
+// AUTO on a rank with no tuned entry and a selector attached: take the
+// selector's pick when its predicted margin is at least min_margin (and keep
+// it for this rank), else run the sweep, which records the measured best.
+// Under stream capture neither may run (both synchronise), so the structural
+// heuristic answers instead.
+static int auto_with_selector(tentaco_csf* h, int R, const float* const* factors_dev,
+                              float* A_dev, int32_t a_rows, hipStream_t st, int* out) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  TT_HIP_CHECK(hipStreamIsCapturing(st, &cs));
+  if (cs != hipStreamCaptureStatusNone) {
+    *out = auto_variant(h->stats, R);
+    return TENTACO_OK;
+  }
+  float f[TENTACO_NUM_FEATURES];
+  int rc = csf_features(h, R, f);
+  if (rc) return rc;
+  int pick = -1;
+  float margin = 0.f;
+  rc = mlp_select(h->selector, h->order, R, f, &pick, &margin, nullptr);
+  if (rc) return rc;
+  if (margin >= h->min_margin) {
+    h->tune.set(R, pick, TENTACO_TUNED_SELECTOR);
+    *out = pick;
+    return TENTACO_OK;
+  }
+  int best = -1;
+  rc = tentaco_csf_autotune(h, R, factors_dev, A_dev, a_rows, 3, st, &best, nullptr);
+  if (rc) return rc;
+  *out = best;
+  return TENTACO_OK;
+}
+
 int tentaco_mttkrp_dev(tentaco_csf* h, int R, const float* const* factors_dev, float* A_dev,
                        int32_t a_rows, int variant, void* stream) {
   if (!h) return fail(TENTACO_EINVAL, "handle is NULL");
@@ -407,23 +452,39 @@ int tentaco_mttkrp_dev(tentaco_csf* h, int R, const float* const* factors_dev, f
     if (!factors_dev[m - 1]) return fail(TENTACO_EINVAL, "factor of level %d is NULL", m);
     F.M[m] = factors_dev[m - 1];
   }
-  const int v = resolve_variant(h, R, variant);
+  int v = -1;
+  int rc = TENTACO_OK;
+  if (variant == TENTACO_VARIANT_AUTO && h->tune.lookup(R) < 0 && h->selector) {
+    rc = auto_with_selector(h, R, factors_dev, A_dev, a_rows, static_cast<hipStream_t>(stream), &v);
+    if (rc) return rc;
+  } else {
+    v = resolve_variant(h, R, variant);
+  }
   const Variant vr = variant_of(v);
   if (vr.family < 0) return fail(TENTACO_EINVAL, "variant %d out of range", variant);
   const CsfDev B = h->view();
-  int rc = reserve_ws(h, mttkrp_workspace_bytes(B, R, vr));
+  rc = reserve_ws(h, mttkrp_workspace_bytes(B, R, vr));
   if (rc) return rc;
   const bool dense = h->dense_root && a_rows == h->nodes[0];
   return launch_mttkrp(B, F, R, A_dev, a_rows, dense, vr, h->ws,
                        static_cast<hipStream_t>(stream));
 }
 
-int tentaco_csf_autotune(tentaco_csf* h, int R, const float* const* factors_dev, float* A_dev,
-                         int32_t a_rows, int reps, void* stream, int* best, float* times_ms) {
+int tentaco_csf_sweep(tentaco_csf* h, int R, const float* const* factors_dev, float* A_dev,
+                      int32_t a_rows, int reps, float cut, void* stream, float* times_ms) {
   clear_error();
   if (!h) return fail(TENTACO_EINVAL, "handle is NULL");
+  if (!times_ms) return fail(TENTACO_EINVAL, "times_ms is NULL");
   if (reps < 1) reps = 1;
   hipStream_t st = static_cast<hipStream_t>(stream);
+  // reserve the largest tile's workspace first: no timed call allocates
+  {
+    size_t need = 0;
+    for (int v = 0; v < kNumVariants; ++v)
+      need = std::max(need, mttkrp_workspace_bytes(h->view(), R, variant_of(v)));
+    int rc = reserve_ws(h, need);
+    if (rc) return rc;
+  }
   hipEvent_t e0, e1;
   TT_HIP_CHECK(hipEventCreate(&e0));
   TT_HIP_CHECK(hipEventCreate(&e1));
@@ -434,43 +495,98 @@ int tentaco_csf_autotune(tentaco_csf* h, int R, const float* const* factors_dev,
       (void)hipEventDestroy(b);
     }
   } ev{e0, e1};
-  int best_v = -1;
-  float best_ms = 0.f;
+  // time `calls` back-to-back calls of variant v between one event pair
+  auto timed = [&](int v, int calls, float* ms) -> int {
+    TT_HIP_CHECK(hipEventRecord(e0, st));
+    for (int c = 0; c < calls; ++c) {
+      int rc = tentaco_mttkrp_dev(h, R, factors_dev, A_dev, a_rows, v, stream);
+      if (rc) return rc;
+    }
+    TT_HIP_CHECK(hipEventRecord(e1, st));
+    TT_HIP_CHECK(hipEventSynchronize(e1));
+    float t = 0.f;
+    TT_HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
+    *ms = t / float(calls);
+    return TENTACO_OK;
+  };
+  // phase 1: a timed warm-up per variant; when it is short (< 2 ms, where
+  // first-launch costs would show) a second call is the probe
+  float best_probe = -1.f;
+  for (int v = 0; v < kNumVariants; ++v) {
+    times_ms[v] = -1.f;
+    float p = 0.f;
+    int rc = timed(v, 1, &p);
+    if (rc == TENTACO_EUNSUPPORTED) {
+      clear_error();
+      continue;
+    }
+    if (rc) return rc;
+    if (p < 2.f) {
+      rc = timed(v, 1, &p);
+      if (rc) return rc;
+    }
+    times_ms[v] = p;
+    if (best_probe < 0.f || p < best_probe) best_probe = p;
+  }
+  if (best_probe < 0.f) return fail(TENTACO_EUNSUPPORTED, "no variant ran");
+  // phase 2: the median of `reps` repetitions for every variant within the
+  // cut; a repetition spans about 100 us (up to 32 calls) so that launch
+  // latency does not dominate small tensors
   std::vector<float> t(static_cast<size_t>(reps));
   for (int v = 0; v < kNumVariants; ++v) {
-    float med = -1.f;
-    int rc = tentaco_mttkrp_dev(h, R, factors_dev, A_dev, a_rows, v, stream);  // warm-up + ws
-    if (rc == TENTACO_OK) {
-      for (int r = 0; r < reps && rc == TENTACO_OK; ++r) {
-        TT_HIP_CHECK(hipEventRecord(e0, st));
-        rc = tentaco_mttkrp_dev(h, R, factors_dev, A_dev, a_rows, v, stream);
-        TT_HIP_CHECK(hipEventRecord(e1, st));
-        TT_HIP_CHECK(hipEventSynchronize(e1));
-        TT_HIP_CHECK(hipEventElapsedTime(&t[size_t(r)], e0, e1));
-      }
-      if (rc == TENTACO_OK) {
-        std::sort(t.begin(), t.end());
-        med = t[t.size() / 2];
-      }
+    const float p = times_ms[v];
+    if (p < 0.f) continue;
+    if (cut > 0.f && p > cut * best_probe) continue;
+    int calls = int(std::ceil(0.1f / std::max(p, 1e-4f)));
+    calls = std::min(32, std::max(1, calls));
+    for (int r = 0; r < reps; ++r) {
+      int rc = timed(v, calls, &t[size_t(r)]);
+      if (rc) return rc;
     }
-    if (rc == TENTACO_EUNSUPPORTED) clear_error();
-    else if (rc != TENTACO_OK) return rc;
-    if (times_ms) times_ms[v] = med;
-    if (med >= 0.f && (best_v < 0 || med < best_ms)) {
-      best_v = v;
-      best_ms = med;
-    }
+    std::sort(t.begin(), t.end());
+    times_ms[v] = t[t.size() / 2];
   }
+  return TENTACO_OK;
+}
+
+int tentaco_csf_autotune(tentaco_csf* h, int R, const float* const* factors_dev, float* A_dev,
+                         int32_t a_rows, int reps, void* stream, int* best, float* times_ms) {
+  std::vector<float> t(static_cast<size_t>(kNumVariants), -1.f);
+  int rc = tentaco_csf_sweep(h, R, factors_dev, A_dev, a_rows, reps, 4.0f, stream, t.data());
+  if (rc) return rc;
+  int best_v = -1;
+  for (int v = 0; v < kNumVariants; ++v)
+    if (t[size_t(v)] >= 0.f && (best_v < 0 || t[size_t(v)] < t[size_t(best_v)])) best_v = v;
   if (best_v < 0) return fail(TENTACO_EUNSUPPORTED, "no variant ran");
-  h->tune.set(R, best_v);
+  h->tune.set(R, best_v, TENTACO_TUNED_SWEEP);
   if (best) *best = best_v;
+  if (times_ms) std::copy(t.begin(), t.end(), times_ms);
   return TENTACO_OK;
 }
 
 int tentaco_csf_set_tuned(tentaco_csf* h, int R, int variant) {
   if (!h) return fail(TENTACO_EINVAL, "handle is NULL");
   if (variant_of(variant).family < 0) return fail(TENTACO_EINVAL, "variant %d out of range", variant);
-  h->tune.set(R, variant);
+  h->tune.set(R, variant, TENTACO_TUNED_SET);
+  return TENTACO_OK;
+}
+
+int tentaco_csf_tuned(const tentaco_csf* h, int R, int* variant, int* source) {
+  if (!h) return fail(TENTACO_EINVAL, "handle is NULL");
+  int src = 0;
+  const int v = h->tune.lookup(R, &src);
+  if (variant) *variant = v;
+  if (source) *source = v < 0 ? 0 : src;
+  return TENTACO_OK;
+}
+
+int tentaco_csf_set_selector(tentaco_csf* h, tentaco_mlp* m, float min_margin) {
+  if (!h) return fail(TENTACO_EINVAL, "handle is NULL");
+  if (m && tentaco_mlp_num_params(kNumVariants) <= 0)
+    return fail(TENTACO_EINVAL, "selector is malformed");
+  if (!(min_margin >= 0.f)) return fail(TENTACO_EINVAL, "min_margin must be >= 0");
+  h->selector = m;
+  h->min_margin = min_margin;
   return TENTACO_OK;
 }
 

@@ -18,6 +18,8 @@ struct tentaco_csf {
   size_t ws_bytes = 0;
   tt::TuneTable tune;  // per-rank best variant found by the autotuner
   tt::CsfStats stats;
+  tentaco_mlp* selector = nullptr;  // learned selector for AUTO (not owned)
+  float min_margin = 0.05f;         // below it AUTO runs the sweep instead
 
   tt::CsfDev view() const {
     tt::CsfDev v;
@@ -34,3 +36,11 @@ struct tentaco_csf {
   }
 };
 
+namespace tt {
+// The 24 selector features of a handle at rank R.
+int csf_features(const tentaco_csf* h, int R, float* features);
+// The selector's pick for an order-`order` tensor at rank R (variants that do
+// not fit are skipped); margin = predicted time of the runner-up / best - 1.
+int mlp_select(tentaco_mlp* m, int order, int R, const float* features, int* variant,
+               float* margin, float* pred_ms);
+}  // namespace tt

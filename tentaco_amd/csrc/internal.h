@@ -101,6 +101,10 @@ int batch_occupancy(int family, int block, int R, int32_t NB, int* per_cu);
 // safe inside captured graphs).
 int launch_fill(float* p, int64_t n, float value, hipStream_t stream);
 
+// Whether variant v's tile fits the LDS limit at this order and rank (the
+// launch refuses it with TENTACO_EUNSUPPORTED otherwise).
+bool variant_fits(int order, int R, int v);
+
 // Event timing of the main kernel of a call (no-op unless the kernel timer
 // is on): begin before the launch, end after it, on the launch stream.
 void ktimer_begin(hipStream_t stream);

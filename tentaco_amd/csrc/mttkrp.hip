@@ -908,6 +908,16 @@ int64_t nnz_per_block(const Variant& vr, int R) {
   return int64_t(vr.block / lanes_per_row(R)) * vr.group;
 }
 
+bool variant_fits(int order, int R, int v) {
+  const Variant vr = variant_of(v);
+  if (vr.family < 0 || R <= 0) return false;
+  if (vr.family == kSliceRow) return true;
+  const int64_t NB = nnz_per_block(vr, R);
+  const size_t stage = edge_stage_ints(order, int32_t(NB)) * sizeof(int32_t);
+  if (vr.family == kEdgeSerial) return stage <= 65536;
+  return edge_ep_bytes(vr.block, lanes_per_row(R), R) + stage <= 65536;
+}
+
 size_t mttkrp_workspace_bytes(const CsfDev& B, int R, const Variant& vr) {
   if (vr.family == kSliceRow || vr.family < 0) return 0;
   const int64_t NB = nnz_per_block(vr, R);

@@ -9,24 +9,31 @@
 namespace tt {
 
 // Best variant per rank found by timing every variant on this tensor.
+// source: how the entry was chosen (TENTACO_TUNED_SWEEP, _SELECTOR, _SET).
 struct TuneTable {
   static constexpr int kSlots = 8;
   int rank[kSlots] = {0, 0, 0, 0, 0, 0, 0, 0};
   int variant[kSlots] = {-1, -1, -1, -1, -1, -1, -1, -1};
-  int lookup(int R) const {
+  int source[kSlots] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int lookup(int R, int* src = nullptr) const {
     for (int i = 0; i < kSlots; ++i)
-      if (rank[i] == R && variant[i] >= 0) return variant[i];
+      if (rank[i] == R && variant[i] >= 0) {
+        if (src) *src = source[i];
+        return variant[i];
+      }
     return -1;
   }
-  void set(int R, int v) {
+  void set(int R, int v, int src) {
     for (int i = 0; i < kSlots; ++i)
       if (rank[i] == R || variant[i] < 0) {
         rank[i] = R;
         variant[i] = v;
+        source[i] = src;
         return;
       }
     rank[kSlots - 1] = R;
     variant[kSlots - 1] = v;
+    source[kSlots - 1] = src;
   }
 };
 
@@ -36,6 +43,10 @@ struct CsfStats {
   int32_t n0 = 0;
   int32_t nonempty_slices = 0;
   int64_t max_slice_nnz = 0;
+  // children per level m-1 node at level m (m = 1..order-1): mean, variance, max
+  double child_mean[kMaxOrder] = {};
+  double child_var[kMaxOrder] = {};
+  double child_max[kMaxOrder] = {};
 };
 
 // Heuristic choice when the handle has not been autotuned: slice-row when the

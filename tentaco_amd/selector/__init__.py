@@ -1,0 +1,239 @@
+"""Learned variant selector: data collection, training and evaluation.
+
+The sweep (``DeviceCsf.sweep``) replaces the reference's codegen.py +
+``*_tune.sh`` loops (codegen.py:44-51, ebsr_32_tune.sh:1-16): it times every
+variant of the table on one tensor. This module runs that sweep over
+synthetic tensors to build the training set, trains the library's MLP on it
+with the library's own HIP kernels (``Selector.train``), and scores the
+trained selector on held-out tensors against the sweep's best, the fixed
+default variant and the best single fixed variant.
+
+Tensors (index i, seed 1004): order U{3,4,5}; per-mode dims log-U[1e3, 1e6];
+a nonzero target log-U[1e4, 1e8]; Zipf slice exponent U[0.5, 2.0]; leaf
+fibre log-normal sigma U[0.3, 2.0]; R from {16, 32, 64}. The generator stops
+at the target or when the slices are exhausted, so small-dim tensors can
+hold fewer nonzeros than the target; the features use the actual counts.
+Indices 0..1999 train, 2000..2199 are held out.
+
+CLI (one GPU):
+  python -m tentaco_amd.selector collect --start 0 --count 500 --out FILE.npz
+  python -m tentaco_amd.selector train --data A.npz B.npz ... --out model.npz
+  python -m tentaco_amd.selector eval --model model.npz --data HELD.npz
+"""
+
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+import time
+from typing import Dict, List, Sequence
+
+import numpy as np
+
+import tentaco_amd as tt
+
+SEED = 1004
+N_TRAIN = 2000
+N_HELD = 200
+MIN_MARGIN = 0.05
+HERE = os.path.dirname(os.path.abspath(__file__))
+DATA_DIR = os.path.join(HERE, "data")
+MODEL_PATH = os.path.join(HERE, "model.npz")
+
+
+def spec(i: int, seed: int = SEED) -> Dict:
+    rng = np.random.default_rng([seed, i])
+    order = int(rng.integers(3, 6))
+    dims = [int(round(10 ** rng.uniform(3, 6))) for _ in range(order)]
+    nnz = int(round(10 ** rng.uniform(4, 8)))
+    s = float(rng.uniform(0.5, 2.0))
+    sigma = float(rng.uniform(0.3, 2.0))
+    R = int(rng.choice([16, 32, 64]))
+    return {"order": order, "dims": dims, "nnz": nnz, "slice_s": s, "leaf_sigma": sigma,
+            "R": R, "seed": seed * 100000 + i}
+
+
+def make_tensor(sp: Dict) -> tt.Csf:
+    return tt.Csf.synth(sp["order"], sp["dims"], sp["nnz"], slice_s=sp["slice_s"], fanout_s=1.3,
+                        mid_mu=1.0, mid_sigma=1.0, leaf_mu=1.5, leaf_sigma=sp["leaf_sigma"],
+                        leaf_cap=10000, seed=sp["seed"])
+
+
+def _save(path, rows):
+    if not rows:
+        return
+    keys = rows[0].keys()
+    np.savez_compressed(path, **{k: np.array([r[k] for r in rows]) for k in keys})
+
+
+def collect(start: int, count: int, out: str, reps: int = 3, cut: float = 4.0,
+            time_limit: float = 1e9, log=print) -> int:
+    """Sweep tensors start..start+count-1 on the GPU; returns how many were
+    done (stops early at time_limit seconds and saves what it has)."""
+    t_start = time.time()
+    rows: List[Dict] = []
+    nv = len(tt.variants())
+    for i in range(start, start + count):
+        if time.time() - t_start > time_limit:
+            break
+        sp = spec(i)
+        t0 = time.time()
+        B = make_tensor(sp)
+        R = sp["R"]
+        H = tt.DeviceCsf(B)
+        rng = np.random.default_rng(i)
+        fd = [tt.DeviceArray.from_numpy(rng.uniform(-1, 1, (B.dims[m], R)).astype(np.float32))
+              for m in range(1, B.order)]
+        A = tt.DeviceArray.empty((B.dims[0], R))
+        times = np.array(H.sweep(fd, A, R, reps=reps, cut=cut), np.float32)
+        feats = H.features(R)
+        nodes = [B.nodes(m) for m in range(B.order)] + [0] * (5 - B.order)
+        dims = list(B.dims) + [0] * (5 - B.order)
+        H.close()
+        for f in fd:
+            f.free()
+        A.free()
+        ok = times > 0
+        best = int(np.argmin(np.where(ok, times, np.inf)))
+        rows.append({"index": i, "order": B.order, "R": R, "nnz": B.nnz, "nnz_target": sp["nnz"],
+                     "dims": np.array(dims, np.int64), "nodes": np.array(nodes, np.int64),
+                     "slice_s": sp["slice_s"], "leaf_sigma": sp["leaf_sigma"],
+                     "features": feats, "times_ms": times, "wall_s": time.time() - t0})
+        log(f"[collect] {i} order={B.order} nnz={B.nnz} R={R} best={tt.variants()[best]} "
+            f"{times[best]:.4f} ms ({time.time() - t0:.2f} s)")
+        if len(rows) % 50 == 0:
+            _save(out, rows)
+        del B
+    _save(out, rows)
+    assert all(len(r["times_ms"]) == nv for r in rows)
+    return len(rows)
+
+
+def load(paths: Sequence[str]) -> Dict[str, np.ndarray]:
+    parts = [np.load(p, allow_pickle=False) for p in paths]
+    keys = parts[0].files
+    d = {k: np.concatenate([p[k] for p in parts]) for k in keys}
+    order = np.argsort(d["index"], kind="stable")
+    d = {k: v[order] for k, v in d.items()}
+    _, first = np.unique(d["index"], return_index=True)
+    return {k: v[first] for k, v in d.items()}
+
+
+def split(d: Dict[str, np.ndarray]):
+    tr = d["index"] < N_TRAIN
+    return ({k: v[tr] for k, v in d.items()}, {k: v[~tr] for k, v in d.items()})
+
+
+def targets(d):
+    t = d["times_ms"].astype(np.float64)
+    mask = (t > 0).astype(np.float32)
+    logt = np.where(t > 0, np.log(np.where(t > 0, t, 1.0)), 0.0).astype(np.float32)
+    return logt, mask
+
+
+def fit(train: Dict[str, np.ndarray], epochs: int = 400, batch: int = 128, lr: float = 3e-3,
+        seed: int = 7) -> "tt.Selector":
+    """Train a selector on the GPU (library kernels) on a sweep data set."""
+    x = train["features"].astype(np.float32)
+    t, mask = targets(train)
+    xm = x.mean(0)
+    xs = np.maximum(x.std(0), 1e-3)
+    valid = t[mask > 0]
+    sel = tt.Selector(t.shape[1], seed=seed)
+    sel.set_scaling(xm, xs, float(valid.mean()), float(max(valid.std(), 1e-3)))
+    sel.final_loss = sel.train(x, t, mask, epochs=epochs, batch=batch, lr=lr, seed=seed)
+    return sel
+
+
+def evaluate(sel: "tt.Selector", d: Dict[str, np.ndarray], min_margin: float = MIN_MARGIN,
+             fixed: int = -1) -> Dict:
+    """Score picks against each tensor's sweep: ratio = time(pick) / time(best)."""
+    names = tt.variants()
+    default = tt.default_variant()
+    times = d["times_ms"].astype(np.float64)
+    ok = times > 0
+    best = np.where(ok, times, np.inf).min(1)
+    model_r, final_r, default_r, fixed_r, margins = [], [], [], [], []
+    fallbacks = 0
+    for k in range(times.shape[0]):
+        v, margin, _ = sel.select(int(d["order"][k]), int(d["R"][k]), d["features"][k])
+        r = times[k, v] / best[k] if ok[k, v] else np.inf
+        model_r.append(r)
+        margins.append(margin)
+        if margin < min_margin:
+            fallbacks += 1
+            final_r.append(1.0)  # the sweep's own best
+        else:
+            final_r.append(r)
+        default_r.append(times[k, default] / best[k] if ok[k, default] else np.inf)
+        if fixed >= 0:
+            fixed_r.append(times[k, fixed] / best[k] if ok[k, fixed] else np.inf)
+    model_r, final_r, default_r = map(np.array, (model_r, final_r, default_r))
+
+    def summ(r):
+        fin = np.isfinite(r)
+        return {"within_10pct": float(np.mean(r <= 1.10)),
+                "geomean_ratio": float(np.exp(np.mean(np.log(np.where(fin, r, 1e3))))),
+                "worst_ratio": float(np.max(np.where(fin, r, np.nan))) if fin.any() else None}
+
+    out = {"n": int(times.shape[0]), "min_margin": min_margin,
+           "selector_with_fallback": summ(final_r), "selector_only": summ(model_r),
+           "fallback_rate": fallbacks / max(1, times.shape[0]),
+           "fixed_default": {"variant": names[default], **summ(default_r)}}
+    if fixed >= 0:
+        out["best_fixed"] = {"variant": names[fixed], **summ(np.array(fixed_r))}
+    return out
+
+
+def best_fixed_variant(train: Dict[str, np.ndarray]) -> int:
+    """The single variant with the smallest geometric-mean ratio to the
+    sweep's best over the training set (unsupported counts as 1000x)."""
+    t = train["times_ms"].astype(np.float64)
+    best = np.where(t > 0, t, np.inf).min(1, keepdims=True)
+    r = np.where(t > 0, t / best, 1e3)
+    return int(np.argmin(np.exp(np.log(r).mean(0))))
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(prog="python -m tentaco_amd.selector")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    c = sub.add_parser("collect")
+    c.add_argument("--start", type=int, required=True)
+    c.add_argument("--count", type=int, required=True)
+    c.add_argument("--out", required=True)
+    c.add_argument("--reps", type=int, default=3)
+    c.add_argument("--cut", type=float, default=4.0)
+    c.add_argument("--time-limit", type=float, default=1e9)
+    t = sub.add_parser("train")
+    t.add_argument("--data", nargs="+", required=True)
+    t.add_argument("--out", default=MODEL_PATH)
+    t.add_argument("--epochs", type=int, default=400)
+    e = sub.add_parser("eval")
+    e.add_argument("--model", default=MODEL_PATH)
+    e.add_argument("--data", nargs="+", required=True)
+    a = ap.parse_args(argv)
+    tt.device_check()
+    if a.cmd == "collect":
+        n = collect(a.start, a.count, a.out, a.reps, a.cut, a.time_limit,
+                    log=lambda s: print(s, flush=True))
+        print(f"[collect] wrote {n} tensors to {a.out}", flush=True)
+    elif a.cmd == "train":
+        d = load(a.data)
+        tr, held = split(d)
+        sel = fit(tr, epochs=a.epochs)
+        fixed = best_fixed_variant(tr)
+        sel.save(a.out, best_fixed=np.int32(fixed), final_loss=np.float32(sel.final_loss))
+        print({"train": int(tr["index"].size), "final_loss": sel.final_loss}, flush=True)
+        if held["index"].size:
+            print(evaluate(sel, held, fixed=fixed), flush=True)
+    else:
+        sel = tt.Selector.load(a.model)
+        d = load(a.data)
+        _, held = split(d)
+        fixed = int(np.load(a.model)["best_fixed"])
+        print(evaluate(sel, held if held["index"].size else d, fixed=fixed), flush=True)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

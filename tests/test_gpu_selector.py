@@ -1,0 +1,176 @@
+"""The learned selector's HIP kernels against the FP64 oracle (forward,
+gradients, Adam), training determinism, the CSF features, and AUTO with a
+selector attached (its pick, or the sweep when the margin is small)."""
+
+import numpy as np
+import pytest
+
+import oracle
+import tentaco_amd as tt
+from helpers import REL_L2_TOL, factors_for, random_csf
+from oracle import mlp
+
+pytestmark = pytest.mark.gpu
+
+
+def _scaled_model(K, seed):
+    rng = np.random.default_rng(seed)
+    sel = tt.Selector(K, seed=seed)
+    xm = rng.normal(0, 1, mlp.IN).astype(np.float32)
+    xs = rng.uniform(0.5, 2.0, mlp.IN).astype(np.float32)
+    sel.set_scaling(xm, xs, 0.25, 1.5)
+    return sel, xm, xs, 0.25, 1.5
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def test_forward_kernel_matches_oracle(gpu):
+    K = tt.lib().tentaco_num_variants()
+    sel, xm, xs, ym, ys = _scaled_model(K, 1)
+    x = np.random.default_rng(2).normal(0, 2, (37, mlp.IN)).astype(np.float32)
+    got = sel.forward(x)
+    ref = mlp.forward(sel.params(), K, x, xm, xs, ym, ys)
+    assert got.shape == (37, K)
+    assert _rel(got, ref) <= 1e-5
+
+
+@pytest.mark.parametrize("K,n", [(33, 6), (7, 1), (64, 50)])
+def test_gradients_match_oracle(gpu, K, n):
+    sel, xm, xs, ym, ys = _scaled_model(K, 3 + K)
+    rng = np.random.default_rng(K * n)
+    x = rng.normal(0, 2, (n, mlp.IN)).astype(np.float32)
+    t = rng.normal(0.5, 1.5, (n, K)).astype(np.float32)
+    mask = (rng.random((n, K)) < 0.85).astype(np.float32)
+    mask[0, 0] = 1.0
+    loss = sel.loss_grad(x, t, mask)
+    g = sel.grads()
+    ref_loss, ref_g = mlp.loss_and_grad(sel.params(), K, x, t, mask, xm, xs, ym, ys)
+    assert loss == pytest.approx(ref_loss, rel=1e-5)
+    # every parameter block separately (a transposed block would pass a
+    # whole-vector norm test only by accident)
+    sizes = [mlp.HID * mlp.IN, mlp.HID, mlp.HID * mlp.HID, mlp.HID, K * mlp.HID, K]
+    o = 0
+    for s in sizes:
+        assert _rel(g[o:o + s], ref_g[o:o + s]) <= 1e-4, (o, s)
+        o += s
+
+
+def test_adam_kernel_matches_oracle(gpu):
+    K = 33
+    sel, xm, xs, ym, ys = _scaled_model(K, 9)
+    rng = np.random.default_rng(10)
+    x = rng.normal(size=(8, mlp.IN)).astype(np.float32)
+    t = rng.normal(size=(8, K)).astype(np.float32)
+    mask = np.ones((8, K), np.float32)
+    p0 = sel.params().astype(np.float64)
+    m = np.zeros_like(p0)
+    v = np.zeros_like(p0)
+    p = p0
+    for step in (1, 2):
+        sel.loss_grad(x, t, mask)
+        g = sel.grads().astype(np.float64)
+        sel.adam(1e-3, 0.9, 0.999, 1e-8)
+        p, m, v = mlp.adam(p, g, m, v, step, 1e-3)
+        assert _rel(sel.params() - p0, p - p0) <= 1e-4
+
+
+def test_training_is_deterministic_and_learns(gpu):
+    rng = np.random.default_rng(11)
+    n, K = 300, 5
+    x = rng.normal(size=(n, mlp.IN)).astype(np.float32)
+    w = rng.normal(size=(mlp.IN, K))
+    t = (np.tanh(x @ w / 4) + 0.1 * rng.normal(size=(n, K))).astype(np.float32)
+    mask = np.ones((n, K), np.float32)
+    params = []
+    losses = []
+    for _ in range(2):
+        sel = tt.Selector(K, seed=5)
+        sel.set_scaling(np.zeros(mlp.IN), np.ones(mlp.IN), 0.0, 1.0)
+        l0 = sel.loss_grad(x, t, mask)
+        losses.append((l0, sel.train(x, t, mask, epochs=60, batch=32, lr=3e-3, seed=3)))
+        params.append(sel.params())
+        sel.close()
+    assert np.array_equal(params[0], params[1])
+    l0, l1 = losses[0]
+    assert l1 < 0.3 * l0, losses
+
+
+def _host_features(B, R):
+    f = np.zeros(24)
+    f[0] = np.log(max(B.nnz, 1))
+    for m in range(B.order):
+        f[1 + m] = np.log(B.dims[m])
+        f[6 + m] = np.log(max(B.nodes(m), 1))
+    for m in range(1, min(B.order, 5)):
+        c = np.diff(np.asarray(B.pos[m], np.float64))
+        f[11 + 3 * (m - 1):14 + 3 * (m - 1)] = np.log1p([c.mean(), c.var(), c.max()])
+    f[23] = np.log(R)
+    return f
+
+
+@pytest.mark.parametrize("order", [3, 4, 5])
+def test_features_match_host_statistics(gpu, order):
+    B = tt.Csf.synth(order, [50 + 13 * m for m in range(order)], 20000, slice_s=1.2, seed=order)
+    H = tt.DeviceCsf(B)
+    got = H.features(32)
+    ref = _host_features(B, 32)
+    assert np.allclose(got, ref, rtol=1e-5, atol=1e-6), (got, ref)
+    H.close()
+
+
+def test_auto_uses_selector_pick_or_sweep(gpu):
+    rng = np.random.default_rng(12)
+    B = random_csf(rng, (60, 40, 30), 5000)
+    R = 32
+    facs = factors_for(rng, B, R)
+    fd = [tt.DeviceArray.from_numpy(f) for f in facs]
+    A = tt.DeviceArray.empty((B.dims[0], R))
+    ref = oracle.mttkrp(B, facs, "taco", fp64=True)
+    names = tt.variants()
+    K = len(names)
+    want = names.index("ep_b256_g16")
+    # a selector whose prediction is fixed: W3 = 0, b3 = 0 for `want`, 1 for the rest
+    sel = tt.Selector(K, seed=0)
+    sel.set_scaling(np.zeros(24), np.ones(24), 0.0, 1.0)
+    p = sel.params()
+    p[-K:] = 1.0
+    p[-K - 64 * K:-K] = 0.0
+    p[-K + want] = 0.0
+    sel.set_params(p)
+    v, margin, pred = sel.select(3, R, np.zeros(24, np.float32))
+    assert v == want and margin == pytest.approx(np.e - 1, rel=1e-5)
+    H = tt.DeviceCsf(B)
+    H.set_selector(sel, 0.05)
+    H.mttkrp(fd, A, R, tt.VARIANT_AUTO)
+    assert H.tuned(R) == (want, tt.TUNED_SELECTOR)
+    assert oracle.rel_l2(A.numpy(), ref) <= REL_L2_TOL
+    H.close()
+    # margin below the threshold: the sweep decides
+    H = tt.DeviceCsf(B)
+    H.set_selector(sel, 10.0)
+    H.mttkrp(fd, A, R, tt.VARIANT_AUTO)
+    v2, src = H.tuned(R)
+    assert src == tt.TUNED_SWEEP and 0 <= v2 < K
+    assert oracle.rel_l2(A.numpy(), ref) <= REL_L2_TOL
+    H.close()
+    sel.close()
+
+
+def test_sweep_leaves_choice_unchanged_and_autotune_records(gpu):
+    rng = np.random.default_rng(13)
+    B = random_csf(rng, (80, 50, 40), 8000)
+    R = 16
+    facs = factors_for(rng, B, R)
+    fd = [tt.DeviceArray.from_numpy(f) for f in facs]
+    A = tt.DeviceArray.empty((B.dims[0], R))
+    H = tt.DeviceCsf(B)
+    times = H.sweep(fd, A, R, reps=2, cut=4.0)
+    assert len(times) == len(tt.variants()) and min(t for t in times if t > 0) > 0
+    assert H.tuned(R) == (-1, tt.TUNED_NONE)
+    best, t2 = H.autotune(fd, A, R, reps=2)
+    assert H.tuned(R) == (best, tt.TUNED_SWEEP)
+    H.mttkrp(fd, A, R, tt.VARIANT_AUTO)
+    assert oracle.rel_l2(A.numpy(), oracle.mttkrp(B, facs, "taco", fp64=True)) <= REL_L2_TOL
+    H.close()
