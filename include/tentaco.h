@@ -85,6 +85,17 @@ int tentaco_ttv_n(int order, taco_tensor_t* A, taco_tensor_t* B,
                   taco_tensor_t* c);
 
 /* ---- layer 2: reference symbol names (gpu_kernels.cuh:4-73) ------------ */
+/* The library exports every name of this layer twice: with C linkage (the
+ * declarations below) and with C++ linkage, as gpu_kernels.cuh declares them,
+ * so objects compiled against either header link. The C++ definitions
+ * forward to these three helpers (family < 0 = the fixed default). */
+float tentaco_legacy_timer_ms(void);
+void tentaco_legacy_mttkrp3(int family, int block, int group, const char* name,
+                            taco_tensor_t* A, taco_tensor_t* B, taco_tensor_t* C,
+                            taco_tensor_t* D);
+void tentaco_legacy_ttv3(const char* name, taco_tensor_t* A, taco_tensor_t* B,
+                         taco_tensor_t* C);
+
 /* Milliseconds of the last call's hot region (partition + zero + kernel),
  * hipEvent-timed like GPUTimer (gpu_library.h:949-988); -1 before any call. */
 float get_gpu_timer_result(void);

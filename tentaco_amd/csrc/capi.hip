@@ -631,89 +631,33 @@ int tentaco_ttv_n(int order, taco_tensor_t* A, taco_tensor_t* B, taco_tensor_t* 
 
 float get_gpu_timer_result(void) { return g_last_ms.load(); }
 
-void mttkrp_gpu_taco(taco_tensor_t* A, taco_tensor_t* B, taco_tensor_t* C, taco_tensor_t* D) {
+float tentaco_legacy_timer_ms(void) { return g_last_ms.load(); }
+
+void tentaco_legacy_mttkrp3(int family, int block, int group, const char* name, taco_tensor_t* A,
+                            taco_tensor_t* B, taco_tensor_t* C, taco_tensor_t* D) {
   taco_tensor_t* f[2] = {C, D};
-  legacy_report("mttkrp_gpu_taco", host_mttkrp(3, A, B, f, TENTACO_VARIANT_DEFAULT));
+  const int v = family < 0 ? TENTACO_VARIANT_DEFAULT : variant_index(family, block, group);
+  legacy_report(name ? name : "mttkrp", host_mttkrp(3, A, B, f, v));
+}
+
+void tentaco_legacy_ttv3(const char* name, taco_tensor_t* A, taco_tensor_t* B, taco_tensor_t* C) {
+  legacy_report(name ? name : "ttv", host_ttv(3, A, B, C));
+}
+
+void mttkrp_gpu_taco(taco_tensor_t* A, taco_tensor_t* B, taco_tensor_t* C, taco_tensor_t* D) {
+  tentaco_legacy_mttkrp3(-1, 0, 0, "mttkrp_gpu_taco", A, B, C, D);
 }
 
 void ttv_csf_gpu_taco(taco_tensor_t* A, taco_tensor_t* B, taco_tensor_t* C) {
-  legacy_report("ttv_csf_gpu_taco", host_ttv(3, A, B, C));
+  tentaco_legacy_ttv3("ttv_csf_gpu_taco", A, B, C);
 }
 
-#define TT_LEGACY(name, fam, tb, grp)                                                \
+#define TT_V(name, fam, tb, grp)                                                     \
   void name(taco_tensor_t* A, taco_tensor_t* B, taco_tensor_t* C, taco_tensor_t* D) { \
-    taco_tensor_t* f[2] = {C, D};                                                    \
-    legacy_report(#name, host_mttkrp(3, A, B, f, variant_index(fam, tb, grp)));      \
+    tentaco_legacy_mttkrp3(fam, tb, grp, #name, A, B, C, D);                         \
   }
-// eb_sr_{TB}_{g}_{c}: edge-serial, block TB, run length g
-TT_LEGACY(mttkrp_32_eb_sr_512_4_1, 0, 512, 4)
-TT_LEGACY(mttkrp_32_eb_sr_256_4_1, 0, 256, 4)
-TT_LEGACY(mttkrp_32_eb_sr_256_4_2, 0, 256, 4)
-TT_LEGACY(mttkrp_32_eb_sr_256_4_4, 0, 256, 4)
-TT_LEGACY(mttkrp_32_eb_sr_256_4_8, 0, 256, 4)
-TT_LEGACY(mttkrp_32_eb_sr_256_8_1, 0, 256, 8)
-TT_LEGACY(mttkrp_32_eb_sr_256_8_2, 0, 256, 8)
-TT_LEGACY(mttkrp_32_eb_sr_256_8_4, 0, 256, 8)
-TT_LEGACY(mttkrp_32_eb_sr_256_8_8, 0, 256, 8)
-TT_LEGACY(mttkrp_32_eb_sr_256_16_1, 0, 256, 16)
-TT_LEGACY(mttkrp_32_eb_sr_256_16_2, 0, 256, 16)
-TT_LEGACY(mttkrp_32_eb_sr_256_16_4, 0, 256, 16)
-TT_LEGACY(mttkrp_32_eb_sr_256_16_8, 0, 256, 16)
-TT_LEGACY(mttkrp_32_eb_sr_256_32_1, 0, 256, 32)
-TT_LEGACY(mttkrp_32_eb_sr_256_32_2, 0, 256, 32)
-TT_LEGACY(mttkrp_32_eb_sr_256_32_4, 0, 256, 32)
-TT_LEGACY(mttkrp_32_eb_sr_256_32_8, 0, 256, 32)
-TT_LEGACY(mttkrp_32_eb_sr_512_4_2, 0, 512, 4)
-TT_LEGACY(mttkrp_32_eb_sr_512_4_4, 0, 512, 4)
-TT_LEGACY(mttkrp_32_eb_sr_512_4_8, 0, 512, 4)
-TT_LEGACY(mttkrp_32_eb_sr_512_8_1, 0, 512, 8)
-TT_LEGACY(mttkrp_32_eb_sr_512_8_2, 0, 512, 8)
-TT_LEGACY(mttkrp_32_eb_sr_512_8_4, 0, 512, 8)
-TT_LEGACY(mttkrp_32_eb_sr_512_8_8, 0, 512, 8)
-TT_LEGACY(mttkrp_32_eb_sr_512_16_1, 0, 512, 16)
-TT_LEGACY(mttkrp_32_eb_sr_512_16_2, 0, 512, 16)
-TT_LEGACY(mttkrp_32_eb_sr_512_16_4, 0, 512, 16)
-TT_LEGACY(mttkrp_32_eb_sr_512_16_8, 0, 512, 16)
-TT_LEGACY(mttkrp_32_eb_sr_512_32_1, 0, 512, 32)
-TT_LEGACY(mttkrp_32_eb_sr_512_32_2, 0, 512, 32)
-TT_LEGACY(mttkrp_32_eb_sr_512_32_4, 0, 512, 32)
-TT_LEGACY(mttkrp_32_eb_sr_512_32_8, 0, 512, 32)
-// eb_pr_{TB}_{Nnz}_{gs}: edge-parallel, block TB, reduction group gs
-TT_LEGACY(mttkrp_32_eb_pr_256_64_32_manual, 1, 256, 32)
-TT_LEGACY(mttkrp_64_eb_pr_256_32_32, 1, 256, 32)
-TT_LEGACY(mttkrp_32_eb_pr_256_32_4, 1, 256, 4)
-TT_LEGACY(mttkrp_32_eb_pr_256_32_8, 1, 256, 8)
-TT_LEGACY(mttkrp_32_eb_pr_256_32_16, 1, 256, 16)
-TT_LEGACY(mttkrp_32_eb_pr_256_32_32, 1, 256, 32)
-TT_LEGACY(mttkrp_32_eb_pr_256_64_4, 1, 256, 4)
-TT_LEGACY(mttkrp_32_eb_pr_256_64_8, 1, 256, 8)
-TT_LEGACY(mttkrp_32_eb_pr_256_64_16, 1, 256, 16)
-TT_LEGACY(mttkrp_32_eb_pr_256_64_32, 1, 256, 32)
-TT_LEGACY(mttkrp_32_eb_pr_256_128_4, 1, 256, 4)
-TT_LEGACY(mttkrp_32_eb_pr_256_128_8, 1, 256, 8)
-TT_LEGACY(mttkrp_32_eb_pr_256_128_16, 1, 256, 16)
-TT_LEGACY(mttkrp_32_eb_pr_256_128_32, 1, 256, 32)
-TT_LEGACY(mttkrp_32_eb_pr_256_256_4, 1, 256, 4)
-TT_LEGACY(mttkrp_32_eb_pr_256_256_8, 1, 256, 8)
-TT_LEGACY(mttkrp_32_eb_pr_256_256_16, 1, 256, 16)
-TT_LEGACY(mttkrp_32_eb_pr_256_256_32, 1, 256, 32)
-TT_LEGACY(mttkrp_32_eb_pr_512_64_4, 1, 512, 4)
-TT_LEGACY(mttkrp_32_eb_pr_512_64_8, 1, 512, 8)
-TT_LEGACY(mttkrp_32_eb_pr_512_64_16, 1, 512, 16)
-TT_LEGACY(mttkrp_32_eb_pr_512_64_32, 1, 512, 32)
-TT_LEGACY(mttkrp_32_eb_pr_512_128_4, 1, 512, 4)
-TT_LEGACY(mttkrp_32_eb_pr_512_128_8, 1, 512, 8)
-TT_LEGACY(mttkrp_32_eb_pr_512_128_16, 1, 512, 16)
-TT_LEGACY(mttkrp_32_eb_pr_512_128_32, 1, 512, 32)
-TT_LEGACY(mttkrp_32_eb_pr_512_256_4, 1, 512, 4)
-TT_LEGACY(mttkrp_32_eb_pr_512_256_8, 1, 512, 8)
-TT_LEGACY(mttkrp_32_eb_pr_512_256_16, 1, 512, 16)
-TT_LEGACY(mttkrp_32_eb_pr_512_256_32, 1, 512, 32)
-TT_LEGACY(mttkrp_32_eb_pr_512_512_4, 1, 512, 4)
-TT_LEGACY(mttkrp_32_eb_pr_512_512_8, 1, 512, 8)
-TT_LEGACY(mttkrp_32_eb_pr_512_512_16, 1, 512, 16)
-TT_LEGACY(mttkrp_32_eb_pr_512_512_32, 1, 512, 32)
-#undef TT_LEGACY
+#include "legacy_names.inc"
+#undef TT_V
 
 int tentaco_num_variants(void) { return kNumVariants; }
 

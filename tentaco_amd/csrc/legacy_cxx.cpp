@@ -1,0 +1,35 @@
+// C++-linkage exports of the reference's GPU entry points. gpu_kernels.cuh:4-73
+// declares get_gpu_timer_result, mttkrp_gpu_taco, ttv_csf_gpu_taco and the 66
+// tuning variants without extern "C", so a C++ caller compiled against that
+// header references their mangled names (for example
+// _Z15mttkrp_gpu_tacoP13taco_tensor_tS0_S0_S0_). The definitions below give
+// the library those names too, next to the extern "C" ones of tentaco.h, so
+// such a caller relinks against it without recompiling. Each forwards to a
+// C helper of a different name; this file must not see tentaco.h, whose
+// extern "C" declarations of the same names would clash.
+
+#include "../../include/taco_tensor_t.h"
+
+extern "C" {
+float tentaco_legacy_timer_ms(void);
+void tentaco_legacy_mttkrp3(int family, int block, int group, const char* name, taco_tensor_t* A,
+                            taco_tensor_t* B, taco_tensor_t* C, taco_tensor_t* D);
+void tentaco_legacy_ttv3(const char* name, taco_tensor_t* A, taco_tensor_t* B, taco_tensor_t* C);
+}
+
+float get_gpu_timer_result() { return tentaco_legacy_timer_ms(); }
+
+void mttkrp_gpu_taco(taco_tensor_t* A, taco_tensor_t* B, taco_tensor_t* C, taco_tensor_t* D) {
+  tentaco_legacy_mttkrp3(-1, 0, 0, "mttkrp_gpu_taco", A, B, C, D);
+}
+
+void ttv_csf_gpu_taco(taco_tensor_t* A, taco_tensor_t* B, taco_tensor_t* C) {
+  tentaco_legacy_ttv3("ttv_csf_gpu_taco", A, B, C);
+}
+
+#define TT_V(name, fam, tb, grp)                                                     \
+  void name(taco_tensor_t* A, taco_tensor_t* B, taco_tensor_t* C, taco_tensor_t* D) { \
+    tentaco_legacy_mttkrp3(fam, tb, grp, #name, A, B, C, D);                         \
+  }
+#include "legacy_names.inc"
+#undef TT_V

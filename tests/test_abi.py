@@ -138,3 +138,50 @@ def test_builder_rejects_out_of_range_coords():
     with pytest.raises(tt.TentacoError) as ei:
         tt.Csf.from_coo((4, 4), coords, np.ones(2, np.float32))
     assert ei.value.code == tt.EINVAL
+
+
+def test_cxx_linkage_caller_links_and_runs(tmp_path):
+    """A C++ caller compiled against gpu_kernels.cuh-style declarations (C++
+    linkage, mangled names) links against the library and reaches it."""
+    import re
+    import shutil
+    import subprocess
+
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    inc = os.path.join(ROOT, "tentaco_amd", "csrc", "legacy_names.inc")
+    names = re.findall(r"^TT_V\((\w+),", open(inc).read(), re.M)
+    assert len(names) == 66
+    sig = "(taco_tensor_t *A, taco_tensor_t *B, taco_tensor_t *C, taco_tensor_t *D)"
+    decls = "\n".join(f"void {n}{sig};" for n in names)
+    table = ", ".join(names)
+    src = f"""
+#include <cstdio>
+#include "taco_tensor_t.h"
+float get_gpu_timer_result();
+void mttkrp_gpu_taco{sig};
+void ttv_csf_gpu_taco(taco_tensor_t *A, taco_tensor_t *B, taco_tensor_t *C);
+{decls}
+typedef void (*fn4)(taco_tensor_t*, taco_tensor_t*, taco_tensor_t*, taco_tensor_t*);
+int main() {{
+  fn4 all[] = {{mttkrp_gpu_taco, {table}}};
+  int n = 0;
+  for (fn4 f : all) n += f != nullptr;
+  // a NULL operand is rejected (reported, no exit) before any device work
+  mttkrp_gpu_taco(nullptr, nullptr, nullptr, nullptr);
+  ttv_csf_gpu_taco(nullptr, nullptr, nullptr);
+  std::printf("%d %g\\n", n, get_gpu_timer_result());
+  return 0;
+}}
+"""
+    (tmp_path / "caller.cpp").write_text(src)
+    libdir = os.path.join(ROOT, "tentaco_amd")
+    exe = tmp_path / "caller"
+    subprocess.run([gxx, "-std=c++17", str(tmp_path / "caller.cpp"), "-I", os.path.join(ROOT, "include"),
+                    "-L", libdir, "-ltentaco", f"-Wl,-rpath,{libdir}", "-Wl,-rpath,/opt/rocm/lib",
+                    "-L/opt/rocm/lib", "-o", str(exe)], check=True, capture_output=True, text=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.split()[0] == "67"
+    assert "mttkrp_gpu_taco failed" in out.stderr and "ttv_csf_gpu_taco failed" in out.stderr
