@@ -394,6 +394,19 @@ int tentaco_dev_copy(void* dst, const void* src, size_t bytes, void* stream);
  * (read_tensor / to_taco_tensor / compress_top_level, ds.h:351-436). */
 typedef struct tentaco_host_csf tentaco_host_csf;
 
+/* FROSTT .tns reader (replaces splatt's load path, ds.h:351-366): one
+ * nonzero per line, "i_1 ... i_N value", 1-based indices, '#' comment and
+ * blank lines skipped; N = columns - 1 (2..5). dims_in (optional, N entries
+ * in file mode order) fixes the dimensions, else each is the largest index.
+ * Values are parsed as double and rounded to FP32; duplicates are summed in
+ * file order. mode_order (optional) maps CSF level l to file mode
+ * mode_order[l]; NULL = shortest mode first (ascending dimension, ties by
+ * mode number), the splatt-like ordering. *order_out and mode_order_out
+ * (N entries) report what was used. Malformed lines fail with EINVAL naming
+ * the line. */
+int tentaco_tns_read(const char* path, const int32_t* dims_in, const int* mode_order,
+                     tentaco_host_csf** out, int* order_out, int* mode_order_out);
+
 /* Lexicographic COO -> CSF: coords is order x nnz (mode-major, coords[m*nnz +
  * q]); duplicate coordinates are summed in input order (FP32); the root is
  * dense (pos = {0, dims[0]}, crd = 0..dims[0]-1). */

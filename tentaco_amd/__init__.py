@@ -151,6 +151,8 @@ def lib() -> ctypes.CDLL:
         "tentaco_ktimer_start": (c_int, []),
         "tentaco_ktimer_stop": (c_int, [POINTER(ctypes.c_double), POINTER(c_int), ctypes.c_char_p, c_int]),
         "tentaco_dev_copy": (c_int, [c_void_p, c_void_p, ctypes.c_size_t, c_void_p]),
+        "tentaco_tns_read": (c_int, [ctypes.c_char_p, POINTER(c_int32), POINTER(c_int), POINTER(c_void_p),
+                                     POINTER(c_int), POINTER(c_int)]),
         "tentaco_coo_to_csf": (c_int, [c_int, POINTER(c_int32), c_int64, POINTER(c_int32), POINTER(c_float), POINTER(c_void_p)]),
         "tentaco_synth_csf": (c_int, [POINTER(SynthParams), POINTER(c_void_p)]),
         "tentaco_host_csf_view": (P, [c_void_p]),
@@ -243,6 +245,22 @@ class Csf:
             )
         )
         return _wrap_host_csf(h)
+
+    @classmethod
+    def from_tns(cls, path: str, dims: Optional[Sequence[int]] = None,
+                 mode_order: Optional[Sequence[int]] = None):
+        """Read a FROSTT .tns file (tentaco_tns_read). Returns (csf, mode_order):
+        CSF level l holds file mode mode_order[l]; by default the shortest mode
+        comes first."""
+        h = c_void_p()
+        order = c_int()
+        used = (c_int * 5)()
+        d = None if dims is None else np.ascontiguousarray(dims, dtype=np.int32)
+        mo = None if mode_order is None else (c_int * len(mode_order))(*mode_order)
+        _check(lib().tentaco_tns_read(os.fsencode(path),
+                                      None if d is None else d.ctypes.data_as(POINTER(c_int32)),
+                                      mo, ctypes.byref(h), ctypes.byref(order), used))
+        return _wrap_host_csf(h), [used[l] for l in range(order.value)]
 
     def reorder(self, mode_order: Sequence[int]) -> "Csf":
         """Same tensor with level m = mode mode_order[m] (tentaco_csf_reorder)."""

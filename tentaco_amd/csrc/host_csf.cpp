@@ -447,6 +447,12 @@ int synth(const tentaco_synth_params& P, HostCsf* out) {
 }
 
 }  // namespace
+
+int build_csf_from_coo(int order, const int32_t* dims, int64_t n, const int32_t* coords,
+                       const float* vals, HostCsf* out) {
+  return build_from_coo(order, dims, n, coords, vals, out);
+}
+
 }  // namespace tt
 
 using tt::fail;

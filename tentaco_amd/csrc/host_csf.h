@@ -21,6 +21,11 @@ struct HostCsf {
   void make_view();
 };
 
+// Lexicographic COO -> CSF (dense root); coords mode-major, duplicates summed
+// in input order. Inputs must already be validated (coords inside dims).
+int build_csf_from_coo(int order, const int32_t* dims, int64_t n, const int32_t* coords,
+                       const float* vals, HostCsf* out);
+
 }  // namespace tt
 
 struct tentaco_host_csf {
