@@ -10,8 +10,10 @@ bytes by the average per-step event time of the dominant kernel; the cold
 figure flushes the 256 MB Infinity Cache before every step. The CPU baseline
 is the restated mttkrp3_cpu_taco (FP64, 64-bit indices, reference OpenMP
 schedule) on the host cores of the same machine, min of 25 cold trials
-(resana.py:20). Extra lines cover the 100M-nonzero 3-mode tensor at R=32 and
-the 20M-nonzero TTV.
+(resana.py:20). Extra entries cover the 3/4/5-mode Zipf tensors (100M / 60M /
+30M nonzeros) at R in {16, 32, 64} with the sweep's best and worst variant,
+the 20M-nonzero TTV, and the 64-tensor uneven batch (one persistent launch
+in a hipGraph against one launch per tensor-mode).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
 """
@@ -231,6 +233,64 @@ def run_mttkrp_workload(tt, B, R, seed, K, W, reps=5, check=True, fp32_check=Tru
     return res, facs
 
 
+C3_TENSORS = ((3, 500_000, 100_000_000), (4, 200_000, 60_000_000), (5, 50_000, 30_000_000))
+
+
+def run_c3_matrix(tt, metrics, oracle, K):
+    """3/4/5-mode Zipf tensors (100M / 60M / 30M nonzeros) at R in {16, 32, 64}:
+    the full sweep (best and worst), the fixed default, and K back-to-back
+    steps of the best; parity of the best against the FP64 oracle at R=32."""
+    import numpy as np
+
+    names = tt.variants()
+    default_v = tt.default_variant()
+    out = {}
+    for order, dim, nnz in C3_TENSORS:
+        t0 = time.time()
+        B = metrics.workload_zipf(order, dim, nnz, seed=1002 + order)
+        H = tt.DeviceCsf(B)
+        log(f"C3 order {order}: {B.nnz} nonzeros, nodes {[B.nodes(m) for m in range(order)]} "
+            f"({time.time() - t0:.1f}s)")
+        for R in (16, 32, 64):
+            facs = metrics.factors(B, R, 1003 + R)
+            fd = [tt.DeviceArray.from_numpy(f) for f in facs]
+            A = tt.DeviceArray.empty((B.dims[0], R))
+            times = H.sweep(fd, A, R, reps=2, cut=4.0)
+            ok = [(t, v) for v, t in enumerate(times) if t > 0]
+            best_ms, best = min(ok)
+            worst_ms, worst = max(ok)
+            H.set_tuned(R, best)
+
+            def step():
+                H.mttkrp(fd, A, R, best)
+
+            step()
+            ms, _ = time_steps(step, max(5, K // 5))
+            ms /= max(5, K // 5)
+            balg = metrics.alg_bytes_mttkrp(B, R)
+            rec = {
+                "nnz": B.nnz, "best": names[best], "best_sweep_ms": round(best_ms, 4),
+                "worst": names[worst], "worst_sweep_ms": round(worst_ms, 4),
+                "default": names[default_v], "default_ms": round(times[default_v], 4),
+                "ms_per_step": round(ms, 4), "gbps": round(balg / (ms * 1e-3) / 1e9, 2),
+                "pct_hbm_peak": round(100 * balg / (ms * 1e-3) / 1e9 / metrics.HBM_PEAK_GBPS, 3),
+                "alg_bytes": balg,
+                "gather_model_gbps": round(metrics.gather_bytes_mttkrp(B, R) / (ms * 1e-3) / 1e9, 2),
+            }
+            if R == 32:
+                rec["rel_l2_vs_oracle_fp64"] = oracle.rel_l2(A.numpy(),
+                                                             oracle.mttkrp(B, facs, "taco", fp64=True))
+            out[f"order{order}_r{R}"] = rec
+            log(f"  R={R}: best {rec['best']} {rec['ms_per_step']} ms ({rec['gbps']} GB/s), "
+                f"worst {rec['worst']} {rec['worst_sweep_ms']} ms, default {rec['default_ms']} ms")
+            for f in fd:
+                f.free()
+            A.free()
+        H.close()
+        del B
+    return out
+
+
 def run_batch_workload(tt, metrics, oracle, K):
     """Every mode of 64 uneven tensors (R=32): one persistent launch in a
     hipGraph vs one launch per tensor-mode (each with its own autotuned
@@ -417,22 +477,7 @@ def main():
         "traffic_detail": traffic,
     }
     if not args.quick:
-        log("generating the 500k^3 / 100M-nonzero Zipf tensor")
-        t0 = time.time()
-        B3 = metrics.workload_zipf(3, 500_000, 100_000_000, seed=1002)
-        log(f"generated in {time.time()-t0:.1f}s: nnz={B3.nnz} fibers={B3.nodes(1)}")
-        r3, _ = run_mttkrp_workload(tt, B3, 32, seed=1003, K=max(5, K // 5), W=2, reps=3,
-                                    fp32_check=False)
-        b3 = metrics.alg_bytes_mttkrp(B3, 32)
-        extra["c3_order3_r32"] = {
-            "gbps": round(b3 / (r3["ms_per_step"] * 1e-3) / 1e9, 2),
-            "pct_hbm_peak": round(100 * b3 / (r3["ms_per_step"] * 1e-3) / 1e9 / metrics.HBM_PEAK_GBPS, 3),
-            "alg_bytes": b3,
-            "gather_model_gbps": round(metrics.gather_bytes_mttkrp(B3, 32) / (r3["ms_per_step"] * 1e-3) / 1e9, 2),
-            **{k: v for k, v in r3.items() if k != "sweep"},
-        }
-        log(f"100M 3-mode R=32: {extra['c3_order3_r32']['gbps']} GB/s via {r3['variant']}")
-        del B3
+        extra["c3"] = run_c3_matrix(tt, metrics, oracle, K)
         log("generating the 100k^3 / 20M-nonzero TTV tensor")
         B2 = metrics.workload_zipf(3, 100_000, 20_000_000, seed=1001)
         c = np.random.default_rng(1011).uniform(-1, 1, 100_000).astype(np.float32)

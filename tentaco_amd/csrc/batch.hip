@@ -86,7 +86,7 @@ int configure(tentaco_batch* b, int family, int block, int group) {
   free_tables(b);
   int rc = dev_alloc(&b->d_tasks, size_t(b->ntasks));
   if (!rc) rc = dev_alloc(&b->d_bnd_off, size_t(b->ntasks));
-  if (!rc) rc = dev_alloc(&b->d_item_off, size_t(b->ntasks));
+  if (!rc) rc = dev_alloc(&b->d_item_off, size_t(b->ntasks) + 1);  // + the total
   if (!rc) rc = dev_alloc(&b->d_brk, size_t(brk_off[b->ntasks]));
   if (rc) return rc;
   for (int t = 0; t < b->ntasks; ++t) b->host_tasks[t].brk = b->d_brk + brk_off[t];
@@ -94,7 +94,7 @@ int configure(tentaco_batch* b, int family, int block, int group) {
                          hipMemcpyHostToDevice));
   TT_HIP_CHECK(hipMemcpy(b->d_bnd_off, bnd.data(), sizeof(int64_t) * b->ntasks,
                          hipMemcpyHostToDevice));
-  TT_HIP_CHECK(hipMemcpy(b->d_item_off, item.data(), sizeof(int64_t) * b->ntasks,
+  TT_HIP_CHECK(hipMemcpy(b->d_item_off, item.data(), sizeof(int64_t) * (b->ntasks + 1),
                          hipMemcpyHostToDevice));
   BatchLaunch& L = b->L;
   L.tasks = b->d_tasks;

@@ -86,7 +86,7 @@ struct BatchLaunch {
   int64_t total_fill = 0;
   const int64_t* bnd_off = nullptr;     // device, prefix of (items+1) per task
   int64_t total_bnd = 0;
-  const int64_t* item_off = nullptr;    // device, prefix of items per task
+  const int64_t* item_off = nullptr;    // device, prefix of items per task (ntasks + 1)
   int32_t total_items = 0;
   int32_t* counter = nullptr;           // device work counter
   int family = 1, block = 256, group = 16, R = 32;

@@ -707,14 +707,22 @@ __global__ __launch_bounds__(512) void batch_kernel(const BatchTaskDev* __restri
                                                     int32_t* __restrict__ counter, int Rr,
                                                     int32_t G, int32_t NB) {
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ int32_t s_item;
-  for (;;) {
+  (void)counter;
+  // Items hold equal nonzero counts, so a static cyclic assignment balances
+  // the grid without a shared counter; a workgroup's items increase, so its
+  // task cursor only moves forward (one search for the first item).
+  int32_t t = -1;
+  int64_t t_end = 0;
+  for (int32_t item = blockIdx.x; item < total_items; item += gridDim.x) {
     __syncthreads();  // the previous item's LDS reads are done
-    if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
-    __syncthreads();
-    const int32_t item = __builtin_amdgcn_readfirstlane(s_item);
-    if (item >= total_items) break;
-    const int32_t t = upper_index(item_off, ntasks, item);
+    if (t < 0) {
+      t = upper_index(item_off, ntasks, item);
+      t_end = item_off[t + 1];
+    }
+    while (item >= t_end) {
+      ++t;
+      t_end = item_off[t + 1];
+    }
     const BatchTaskDev& T = tasks[t];
     const int32_t b = item - int32_t(item_off[t]);
     switch (T.B.order) {

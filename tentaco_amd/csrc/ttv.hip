@@ -3,9 +3,11 @@
 // are the innermost fibers. Replaces ttv_kernel / ttv_csf_gpu_taco
 // (tensor_kernels/ttv_csf_gpu_taco.h:2-121).
 //
-// Work split: each workgroup owns 4*BS consecutive nonzeros, each lane 4
-// consecutive ones (one dwordx4 load of crd and of vals per lane, fully
-// coalesced per wavefront). A lane sums its nonzeros per fiber; fibers that
+// Work split: each workgroup owns 8*BS consecutive nonzeros, each lane 8
+// consecutive ones (two dwordx4 loads of crd and of vals per lane, fully
+// coalesced per wavefront). The nonzero loads and the c gathers are issued
+// first, so their latency overlaps the bracket load and the staging of the
+// workgroup's fiber bounds in LDS. A lane sums its nonzeros per fiber; fibers that
 // start and end inside a lane are stored directly, the lane's first and last
 // partial sums are merged across the wavefront by a segmented shuffle scan
 // keyed by fiber id, and the lane holding a fiber's last nonzero writes the
@@ -23,7 +25,7 @@ namespace tt {
 namespace {
 
 constexpr int kTtvBlock = 256;
-constexpr int kTtvPerLane = 4;
+constexpr int kTtvPerLane = 8;
 constexpr int kTtvNB = kTtvBlock * kTtvPerLane;
 
 // y[0..nf) = 0; brk[b] = fiber containing min(b*NB, nnz-1), b in 0..nblk.
@@ -59,6 +61,30 @@ __global__ __launch_bounds__(kTtvBlock) void ttv_kernel(
   const int32_t p0 = P + int32_t(threadIdx.x) * U;
   const int32_t p1 = min(p0 + U, Pend);
   const bool active = p0 < p1;
+  // nonzeros and c gathers first: independent of the fiber bounds
+  int32_t k[U];
+  float v[U];
+  float cg[U];
+  if (active) {
+    if (p0 + U <= nnz) {
+#pragma unroll
+      for (int h = 0; h < U; h += 4) {
+        const int4 kk = *reinterpret_cast<const int4*>(crd + p0 + h);
+        const float4 vv = *reinterpret_cast<const float4*>(vals + p0 + h);
+        k[h] = kk.x; k[h + 1] = kk.y; k[h + 2] = kk.z; k[h + 3] = kk.w;
+        v[h] = vv.x; v[h + 1] = vv.y; v[h + 2] = vv.z; v[h + 3] = vv.w;
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const bool ok = p0 + u < nnz;
+        k[u] = ok ? crd[p0 + u] : 0;
+        v[u] = ok ? vals[p0 + u] : 0.f;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) cg[u] = cv[k[u]];
+  }
   const int32_t lo = brk[b];
   const int32_t hi = brk[b + 1];
   const int32_t nstage = hi - lo + 2;
@@ -80,24 +106,6 @@ __global__ __launch_bounds__(kTtvBlock) void ttv_kernel(
   float sum = 0.f, vf = 0.f;
   bool closed_first = false, ends_here = false, starts_before = false;
   if (active) {
-    int32_t k[U];
-    float v[U];
-    if (p0 + U <= nnz) {
-      const int4 kk = *reinterpret_cast<const int4*>(crd + p0);
-      const float4 vv = *reinterpret_cast<const float4*>(vals + p0);
-      k[0] = kk.x; k[1] = kk.y; k[2] = kk.z; k[3] = kk.w;
-      v[0] = vv.x; v[1] = vv.y; v[2] = vv.z; v[3] = vv.w;
-    } else {
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const bool ok = p0 + u < nnz;
-        k[u] = ok ? crd[p0 + u] : 0;
-        v[u] = ok ? vals[p0 + u] : 0.f;
-      }
-    }
-    float cg[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) cg[u] = cv[k[u]];
     f = search(lo, hi, p0);
     nxt = P_at(f + 1);
     first = f;

@@ -154,13 +154,16 @@ def evaluate(sel: "tt.Selector", d: Dict[str, np.ndarray], min_margin: float = M
     times = d["times_ms"].astype(np.float64)
     ok = times > 0
     best = np.where(ok, times, np.inf).min(1)
-    model_r, final_r, default_r, fixed_r, margins = [], [], [], [], []
+    model_r, final_r, default_r, fixed_r, margins, top4_r = [], [], [], [], [], []
     fallbacks = 0
     for k in range(times.shape[0]):
-        v, margin, _ = sel.select(int(d["order"][k]), int(d["R"][k]), d["features"][k])
+        v, margin, pred = sel.select(int(d["order"][k]), int(d["R"][k]), d["features"][k])
         r = times[k, v] / best[k] if ok[k, v] else np.inf
         model_r.append(r)
         margins.append(margin)
+        # a cheaper fallback: time only the 4 best-predicted variants
+        cand = [c for c in np.argsort(pred) if ok[k, c]][:4]
+        top4_r.append(min(times[k, c] for c in cand) / best[k] if cand else np.inf)
         if margin < min_margin:
             fallbacks += 1
             final_r.append(1.0)  # the sweep's own best
@@ -180,6 +183,7 @@ def evaluate(sel: "tt.Selector", d: Dict[str, np.ndarray], min_margin: float = M
     out = {"n": int(times.shape[0]), "min_margin": min_margin,
            "selector_with_fallback": summ(final_r), "selector_only": summ(model_r),
            "fallback_rate": fallbacks / max(1, times.shape[0]),
+           "selector_top4_timed": summ(np.array(top4_r)),
            "fixed_default": {"variant": names[default], **summ(default_r)}}
     if fixed >= 0:
         out["best_fixed"] = {"variant": names[fixed], **summ(np.array(fixed_r))}
