@@ -180,6 +180,47 @@ def per_step_ms(step, K, cold):
     return out
 
 
+def learned_pick(tt, H, order, R, times):
+    """The trained selector's pick for this tensor and its swept time over
+    the sweep's best (the second half of the headline metric)."""
+    from tentaco_amd import selector as S
+
+    sel = _selector(tt, S)
+    v, margin, _ = sel.select(order, R, H.features(R))
+    ok = [t for t in times if t > 0]
+    best = min(ok)
+    return {"learned_pick": tt.variants()[v], "learned_margin": round(margin, 4),
+            "learned_pick_ms": round(times[v], 5),
+            "learned_over_sweep_best": round(times[v] / best, 4) if times[v] > 0 else None,
+            "would_fall_back_to_sweep": margin < S.MIN_MARGIN}
+
+
+_SEL = {}
+
+
+def _selector(tt, S):
+    if "m" not in _SEL:
+        _SEL["m"] = tt.Selector.load(S.MODEL_PATH)
+    return _SEL["m"]
+
+
+def run_selector_heldout(tt):
+    """The trained selector on the 200 held-out tensors of the committed
+    sweep data (their recorded sweep timings are the reference)."""
+    import glob
+
+    import numpy as np
+
+    from tentaco_amd import selector as S
+
+    d = S.load(sorted(glob.glob(os.path.join(S.DATA_DIR, "sweep_*.npz"))))
+    train, held = S.split(d)
+    fixed = int(np.load(S.MODEL_PATH)["best_fixed"])
+    out = S.evaluate(_selector(tt, S), held, fixed=fixed)
+    out["uniform_family"] = S.evaluate(_selector(tt, S), S.split_uniform_heldout(d), fixed=fixed)
+    return out
+
+
 def run_mttkrp_workload(tt, B, R, seed, K, W, reps=5, check=True, fp32_check=True):
     import numpy as np
 
@@ -223,6 +264,7 @@ def run_mttkrp_workload(tt, B, R, seed, K, W, reps=5, check=True, fp32_check=Tru
         "sweep_best_ms": min(t for t in times if t >= 0),
         "sweep_worst_ms": max(t for t in times if t >= 0),
         "sweep": {names[v]: round(t, 5) for v, t in enumerate(times)},
+        **learned_pick(tt, H, B.order, R, times),
     }
     if check:
         got = A.numpy()
@@ -277,6 +319,7 @@ def run_c3_matrix(tt, metrics, oracle, K):
                 "alg_bytes": balg,
                 "gather_model_gbps": round(metrics.gather_bytes_mttkrp(B, R) / (ms * 1e-3) / 1e9, 2),
             }
+            rec.update(learned_pick(tt, H, order, R, times))
             if R == 32:
                 rec["rel_l2_vs_oracle_fp64"] = oracle.rel_l2(A.numpy(),
                                                              oracle.mttkrp(B, facs, "taco", fp64=True))
@@ -333,6 +376,7 @@ def run_batch_workload(tt, metrics, oracle, K):
     bt = tt.Batch(tasks, R)
     bt.set_operands(facs_d, outs, rows)
     (fam, blk, grp), tune_ms = bt.autotune(reps=3)
+    sched = bt.schedule
     items, grid, nb = bt.items()
     bt.run()
     tt.synchronize()
@@ -371,6 +415,7 @@ def run_batch_workload(tt, metrics, oracle, K):
         "speedup_graph_vs_per_task": round(per_ms / g_ms, 3),
         "batched_gbps": round(balg / (g_ms * 1e-3) / 1e9, 2),
         "config": {"family": ["edge-serial", "edge-parallel"][fam], "block": blk, "group": grp,
+                   "schedule": ["work counter", "static cyclic"][sched],
                    "items": items, "persistent_grid": grid, "nnz_per_item": nb},
         "max_rel_l2_batched_vs_per_task": max(eager_err, graph_err),
         "oracle_rel_l2_smallest_tasks": spot,
@@ -508,6 +553,18 @@ def main():
 
     if not args.quick:
         extra["c4_batch"] = run_batch_workload(tt, metrics, oracle, K)
+    extra["c5_selector_heldout"] = run_selector_heldout(tt)
+    live = [res] + ([v for v in extra["c3"].values()] if "c3" in extra else [])
+    ratios = [r["learned_over_sweep_best"] for r in live if r.get("learned_over_sweep_best")]
+    extra["learned_over_sweep_best"] = {
+        "heldout_with_fallback_geomean": extra["c5_selector_heldout"]["selector_with_fallback"]["geomean_ratio"],
+        "heldout_selector_only_geomean": extra["c5_selector_heldout"]["selector_only"]["geomean_ratio"],
+        "live_bench_tensors_geomean": round(float(np.exp(np.mean(np.log(ratios)))), 4) if ratios else None,
+        "live_bench_tensors": len(ratios),
+    }
+    log(f"selector: held-out within 10% (with fallback) "
+        f"{extra['c5_selector_heldout']['selector_with_fallback']['within_10pct']:.3f}; "
+        f"live learned/sweep-best geomean {extra['learned_over_sweep_best']['live_bench_tensors_geomean']}")
 
     line = {
         "metric": "MTTKRP-3 R=32 GB/s (% HBM peak); learned-pick time / sweep-best time",

@@ -351,12 +351,18 @@ int tentaco_batch_launch(tentaco_batch* b, void* stream);
  * configuration. */
 int tentaco_batch_items(const tentaco_batch* b, int32_t* items, int32_t* grid,
                         int32_t* nb);
-/* Time every tile configuration (2 families x 3 blocks x 5 groups), keep the
- * fastest configured and return it. Not for graph capture. */
+/* Time every tile configuration (2 schedules x 2 families x 3 blocks x 5
+ * groups), keep the fastest configured and return it. Not for graph capture. */
 int tentaco_batch_autotune(tentaco_batch* b, int reps, void* stream,
                            int* best_family, int* best_block, int* best_group,
                            float* best_ms);
 int tentaco_batch_free(tentaco_batch* b);
+/* How persistent workgroups take items: 0 = from a device work counter in
+ * arrival order (the default), 1 = static cyclic assignment (item = block +
+ * k * grid, no shared counter). Kept across configure(); autotune searches
+ * both and leaves the faster set. */
+int tentaco_batch_set_schedule(tentaco_batch* b, int schedule);
+int tentaco_batch_get_schedule(const tentaco_batch* b, int* schedule);
 
 /* Device memory helpers for hosts without their own HIP runtime binding. */
 int tentaco_dev_malloc(void** ptr, size_t bytes);

@@ -140,6 +140,8 @@ def lib() -> ctypes.CDLL:
         "tentaco_batch_items": (c_int, [c_void_p, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32)]),
         "tentaco_batch_autotune": (c_int, [c_void_p, c_int, c_void_p, POINTER(c_int), POINTER(c_int), POINTER(c_int), POINTER(c_float)]),
         "tentaco_batch_free": (c_int, [c_void_p]),
+        "tentaco_batch_set_schedule": (c_int, [c_void_p, c_int]),
+        "tentaco_batch_get_schedule": (c_int, [c_void_p, POINTER(c_int)]),
         "tentaco_dev_malloc": (c_int, [POINTER(c_void_p), c_size_t]),
         "tentaco_dev_free": (c_int, [c_void_p]),
         "tentaco_memcpy_h2d": (c_int, [c_void_p, c_void_p, c_size_t]),
@@ -571,6 +573,16 @@ class Batch:
         i, g, nb = c_int32(), c_int32(), c_int32()
         _check(lib().tentaco_batch_items(self.h, ctypes.byref(i), ctypes.byref(g), ctypes.byref(nb)))
         return i.value, g.value, nb.value
+
+    def set_schedule(self, schedule: int):
+        """0: device work counter (default); 1: static cyclic items."""
+        _check(lib().tentaco_batch_set_schedule(self.h, schedule))
+
+    @property
+    def schedule(self) -> int:
+        s = c_int()
+        _check(lib().tentaco_batch_get_schedule(self.h, ctypes.byref(s)))
+        return s.value
 
     def autotune(self, reps: int = 3, stream=None):
         f, b, g, ms = c_int(), c_int(), c_int(), c_float()

@@ -191,6 +191,21 @@ int tentaco_batch_configure(tentaco_batch* b, int family, int block, int group) 
   return configure(b, family, block, group);
 }
 
+int tentaco_batch_set_schedule(tentaco_batch* b, int schedule) {
+  clear_error();
+  if (!b) return fail(TENTACO_EINVAL, "batch is NULL");
+  if (schedule != 0 && schedule != 1) return fail(TENTACO_EINVAL, "schedule must be 0 or 1");
+  if (b->L.schedule != schedule) drop_graph(b);
+  b->L.schedule = schedule;
+  return TENTACO_OK;
+}
+
+int tentaco_batch_get_schedule(const tentaco_batch* b, int* schedule) {
+  if (!b || !schedule) return fail(TENTACO_EINVAL, "NULL argument");
+  *schedule = b->L.schedule;
+  return TENTACO_OK;
+}
+
 int tentaco_batch_run(tentaco_batch* b, void* stream) {
   if (!b || !b->have_operands) return fail(TENTACO_EINVAL, "batch has no operands");
   return launch_batch(b->L, static_cast<hipStream_t>(stream));
@@ -245,12 +260,14 @@ int tentaco_batch_autotune(tentaco_batch* b, int reps, void* stream, int* best_f
   TT_HIP_CHECK(hipEventCreate(&e0));
   TT_HIP_CHECK(hipEventCreate(&e1));
   float bm = -1.f;
-  int bf = -1, bb = 0, bg = 0;
+  int bf = -1, bb = 0, bg = 0, bs = 0;
   std::vector<float> t(static_cast<size_t>(reps));
+  for (int sched = 0; sched < 2; ++sched)
   for (int fam = 0; fam < 2; ++fam)
     for (int blk : {128, 256, 512})
       for (int grp : {4, 8, 16, 32, 64}) {
         int rc = tentaco_batch_configure(b, fam, blk, grp);
+        if (!rc) rc = tentaco_batch_set_schedule(b, sched);
         if (!rc) rc = tentaco_batch_run(b, stream);  // warm-up
         for (int r = 0; r < reps && !rc; ++r) {
           (void)hipEventRecord(e0, st);
@@ -275,12 +292,14 @@ int tentaco_batch_autotune(tentaco_batch* b, int reps, void* stream, int* best_f
           bf = fam;
           bb = blk;
           bg = grp;
+          bs = sched;
         }
       }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (bf < 0) return fail(TENTACO_EUNSUPPORTED, "no batch configuration ran");
   int rc = tentaco_batch_configure(b, bf, bb, bg);
+  if (!rc) rc = tentaco_batch_set_schedule(b, bs);
   if (rc) return rc;
   if (best_family) *best_family = bf;
   if (best_block) *best_block = bb;

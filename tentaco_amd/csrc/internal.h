@@ -90,6 +90,7 @@ struct BatchLaunch {
   int32_t total_items = 0;
   int32_t* counter = nullptr;           // device work counter
   int family = 1, block = 256, group = 16, R = 32;
+  int schedule = 0;  // 0: device work counter, 1: static cyclic items
   int32_t NB = 0;
   int grid = 0;
 };

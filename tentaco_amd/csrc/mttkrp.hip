@@ -705,16 +705,24 @@ __global__ __launch_bounds__(512) void batch_kernel(const BatchTaskDev* __restri
                                                     const int64_t* __restrict__ item_off,
                                                     int32_t total_items,
                                                     int32_t* __restrict__ counter, int Rr,
-                                                    int32_t G, int32_t NB) {
+                                                    int32_t G, int32_t NB, int dynamic) {
   extern __shared__ __align__(16) unsigned char smem[];
-  (void)counter;
-  // Items hold equal nonzero counts, so a static cyclic assignment balances
-  // the grid without a shared counter; a workgroup's items increase, so its
-  // task cursor only moves forward (one search for the first item).
+  __shared__ int32_t s_item;
+  // schedule 0: items are pulled from the device counter (reset by the prep
+  // launch) in arrival order; schedule 1: static cyclic assignment (item =
+  // block + k * grid), no shared counter. Either way a workgroup's items
+  // increase, so its task cursor only moves forward.
   int32_t t = -1;
   int64_t t_end = 0;
-  for (int32_t item = blockIdx.x; item < total_items; item += gridDim.x) {
+  int32_t item = blockIdx.x;
+  for (;;) {
     __syncthreads();  // the previous item's LDS reads are done
+    if (dynamic) {
+      if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+      __syncthreads();
+      item = __builtin_amdgcn_readfirstlane(s_item);
+    }
+    if (item >= total_items) break;
     if (t < 0) {
       t = upper_index(item_off, ntasks, item);
       t_end = item_off[t + 1];
@@ -731,6 +739,7 @@ __global__ __launch_bounds__(512) void batch_kernel(const BatchTaskDev* __restri
       case 4: edge_block<4, R, EP>(T.B, T.F, T.A, Rr, T.brk, b, G, NB, smem); break;
       default: edge_block<5, R, EP>(T.B, T.F, T.A, Rr, T.brk, b, G, NB, smem); break;
     }
+    if (!dynamic) item += gridDim.x;
   }
 }
 
@@ -751,10 +760,12 @@ int launch_batch_typed(const BatchLaunch& L, hipStream_t st) {
   ktimer_begin(st);
   if (L.family == kEdgeParallel)
     batch_kernel<R, true><<<grid, L.block, lds, st>>>(L.tasks, L.ntasks, L.item_off,
-                                                       L.total_items, L.counter, L.R, L.group, L.NB);
+                                                       L.total_items, L.counter, L.R, L.group, L.NB,
+                                                       L.schedule == 0);
   else
     batch_kernel<R, false><<<grid, L.block, lds, st>>>(L.tasks, L.ntasks, L.item_off,
-                                                        L.total_items, L.counter, L.R, L.group, L.NB);
+                                                        L.total_items, L.counter, L.R, L.group, L.NB,
+                                                        L.schedule == 0);
   TT_HIP_CHECK(hipGetLastError());
   ktimer_end(st, "batch_kernel");
   return TENTACO_OK;

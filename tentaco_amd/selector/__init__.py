@@ -15,6 +15,13 @@ at the target or when the slices are exhausted, so small-dim tensors can
 hold fewer nonzeros than the target; the features use the actual counts.
 Indices 0..1999 train, 2000..2199 are held out.
 
+A second family (indices 10000..10399) has i.i.d. uniform coordinates with
+duplicates summed, like the 4096^3 / 500,000-coordinate benchmark tensor:
+order U{3,4,5}, dims log-U[1e2, 1e5], coordinates log-U[1e4, 1e7], R from
+{16, 32, 64}. Its fibres are short (often one nonzero), a regime the Zipf /
+log-normal generator never produces; 10000..10359 train, 10360..10399 are
+held out separately.
+
 CLI (one GPU):
   python -m tentaco_amd.selector collect --start 0 --count 500 --out FILE.npz
   python -m tentaco_amd.selector train --data A.npz B.npz ... --out model.npz
@@ -36,6 +43,9 @@ import tentaco_amd as tt
 SEED = 1004
 N_TRAIN = 2000
 N_HELD = 200
+UNIFORM_BASE = 10000
+N_UNIFORM_TRAIN = 360
+N_UNIFORM = 400
 MIN_MARGIN = 0.05
 HERE = os.path.dirname(os.path.abspath(__file__))
 DATA_DIR = os.path.join(HERE, "data")
@@ -44,6 +54,13 @@ MODEL_PATH = os.path.join(HERE, "model.npz")
 
 def spec(i: int, seed: int = SEED) -> Dict:
     rng = np.random.default_rng([seed, i])
+    if i >= UNIFORM_BASE:
+        order = int(rng.integers(3, 6))
+        dims = [int(round(10 ** rng.uniform(2, 5))) for _ in range(order)]
+        nnz = int(round(10 ** rng.uniform(4, 7)))
+        R = int(rng.choice([16, 32, 64]))
+        return {"family": "uniform", "order": order, "dims": dims, "nnz": nnz, "slice_s": 0.0,
+                "leaf_sigma": 0.0, "R": R, "seed": seed * 100000 + i}
     order = int(rng.integers(3, 6))
     dims = [int(round(10 ** rng.uniform(3, 6))) for _ in range(order)]
     nnz = int(round(10 ** rng.uniform(4, 8)))
@@ -55,6 +72,11 @@ def spec(i: int, seed: int = SEED) -> Dict:
 
 
 def make_tensor(sp: Dict) -> tt.Csf:
+    if sp.get("family") == "uniform":
+        rng = np.random.default_rng(sp["seed"])
+        coords = np.stack([rng.integers(0, d, sp["nnz"], dtype=np.int64) for d in sp["dims"]])
+        vals = rng.uniform(-1, 1, sp["nnz"]).astype(np.float32)
+        return tt.Csf.from_coo(sp["dims"], coords.astype(np.int32), vals)
     return tt.Csf.synth(sp["order"], sp["dims"], sp["nnz"], slice_s=sp["slice_s"], fanout_s=1.3,
                         mid_mu=1.0, mid_sigma=1.0, leaf_mu=1.5, leaf_sigma=sp["leaf_sigma"],
                         leaf_cap=10000, seed=sp["seed"])
@@ -121,8 +143,17 @@ def load(paths: Sequence[str]) -> Dict[str, np.ndarray]:
 
 
 def split(d: Dict[str, np.ndarray]):
-    tr = d["index"] < N_TRAIN
-    return ({k: v[tr] for k, v in d.items()}, {k: v[~tr] for k, v in d.items()})
+    """(train, held-out Zipf tensors); the uniform family's held-out part is
+    split_uniform_heldout()."""
+    i = d["index"]
+    tr = (i < N_TRAIN) | ((i >= UNIFORM_BASE) & (i < UNIFORM_BASE + N_UNIFORM_TRAIN))
+    he = (i >= N_TRAIN) & (i < N_TRAIN + N_HELD)
+    return ({k: v[tr] for k, v in d.items()}, {k: v[he] for k, v in d.items()})
+
+
+def split_uniform_heldout(d: Dict[str, np.ndarray]):
+    he = d["index"] >= UNIFORM_BASE + N_UNIFORM_TRAIN
+    return {k: v[he] for k, v in d.items()}
 
 
 def targets(d):
@@ -230,7 +261,10 @@ def main(argv=None):
         sel.save(a.out, best_fixed=np.int32(fixed), final_loss=np.float32(sel.final_loss))
         print({"train": int(tr["index"].size), "final_loss": sel.final_loss}, flush=True)
         if held["index"].size:
-            print(evaluate(sel, held, fixed=fixed), flush=True)
+            print({"heldout_zipf": evaluate(sel, held, fixed=fixed)}, flush=True)
+        hu = split_uniform_heldout(d)
+        if hu["index"].size:
+            print({"heldout_uniform": evaluate(sel, hu, fixed=fixed)}, flush=True)
     else:
         sel = tt.Selector.load(a.model)
         d = load(a.data)

@@ -48,16 +48,18 @@ def test_batch_matches_oracle_every_config(gpu):
     refs = [oracle.mttkrp(B, f, "taco", fp64=True) for B, f in zip(hosts, facs_h)]
     bt = tt.Batch(tasks, R)
     bt.set_operands(facs_d, outs, rows)
-    for fam in (0, 1):
-        for blk in (128, 256, 512):
-            for grp in (4, 16, 64):
-                bt.configure(fam, blk, grp)
-                for o in outs:  # poison: every run must overwrite
-                    o.upload(np.full(o.shape, 9.0, np.float32))
-                bt.run()
-                for t, (o, ref) in enumerate(zip(outs, refs)):
-                    e = oracle.rel_l2(o.numpy(), ref)
-                    assert e <= REL_L2_TOL, (fam, blk, grp, t, e)
+    configs = [(sc, f, b, g) for sc in (0, 1) for f in (0, 1)
+               for b in (128, 256, 512) for g in (4, 16, 64)]
+    for sched, fam, blk, grp in configs:
+        bt.configure(fam, blk, grp)
+        bt.set_schedule(sched)
+        assert bt.schedule == sched
+        for o in outs:  # poison: every run must overwrite
+            o.upload(np.full(o.shape, 9.0, np.float32))
+        bt.run()
+        for t, (o, ref) in enumerate(zip(outs, refs)):
+            e = oracle.rel_l2(o.numpy(), ref)
+            assert e <= REL_L2_TOL, (sched, fam, blk, grp, t, e)
 
 
 def test_batch_graph_replays_match_eager(gpu):

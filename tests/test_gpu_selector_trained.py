@@ -22,7 +22,7 @@ def data():
     assert DATA, "selector sweep data missing"
     d = S.load(DATA)
     train, held = S.split(d)
-    assert train["index"].size >= 1900 and held["index"].size >= 190
+    assert train["index"].size >= 2200 and held["index"].size >= 190
     return train, held
 
 
@@ -43,6 +43,17 @@ def test_retrained_selector_meets_target(gpu, data, trained):
     assert r["selector_only"]["within_10pct"] > r["fixed_default"]["within_10pct"], r
     assert r["selector_only"]["within_10pct"] > r["best_fixed"]["within_10pct"], r
     assert r["selector_only"]["geomean_ratio"] < r["best_fixed"]["geomean_ratio"], r
+    assert r["selector_only"]["geomean_ratio"] < r["fixed_default"]["geomean_ratio"], r
+
+
+def test_uniform_family_heldout(gpu, trained):
+    """Short-fibre tensors with uniform coordinates (like the 4096^3
+    benchmark tensor), held out of training."""
+    d = S.load(DATA)
+    hu = S.split_uniform_heldout(d)
+    assert hu["index"].size >= 30
+    r = S.evaluate(trained, hu)
+    assert r["selector_with_fallback"]["within_10pct"] >= 0.90, r
     assert r["selector_only"]["geomean_ratio"] < r["fixed_default"]["geomean_ratio"], r
 
 
