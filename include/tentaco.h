@@ -245,8 +245,35 @@ int tentaco_csf_set_tuned(tentaco_csf* h, int R, int variant);
 #define TENTACO_TUNED_SWEEP 1     /* measured by the sweep (autotune) */
 #define TENTACO_TUNED_SELECTOR 2  /* the learned selector's pick */
 #define TENTACO_TUNED_SET 3       /* set by tentaco_csf_set_tuned */
+#define TENTACO_TUNED_SELECTOR_TIMED 4  /* best of the selector's top-k, timed */
 /* The recorded choice for rank R: *variant = -1 and *source = NONE if none. */
 int tentaco_csf_tuned(const tentaco_csf* h, int R, int* variant, int* source);
+
+/* ---- CP-ALS around the batched MTTKRP ---------------------------------- */
+/* The caller splatt_cpd_als stands in for (ds.h:395), for many tensors at
+ * once. handles[t*5 + m] is tensor t's device CSF with mode m at its root
+ * (NULL for m >= its order); level_modes[(t*5 + m)*5 + l] is the tensor mode
+ * held at level l of that CSF (level 0 must be m). Factors start U[0,1) from
+ * seed (as gen_factor_matrices, ds.h:386-409), lambda = 1. R in {16,32,64}. */
+typedef struct tentaco_cpd tentaco_cpd;
+int tentaco_cpd_create(tentaco_csf* const* handles, const int* level_modes, int ntensors,
+                       int R, uint64_t seed, tentaco_cpd** out);
+/* Autotune the batched MTTKRP of every mode step (not for capture). */
+int tentaco_cpd_tune(tentaco_cpd* c, int reps);
+/* `sweeps` ALS sweeps. Mode step m = 0..4: one batched MTTKRP over every
+ * tensor of order > m (replayed from a hipGraph when use_graphs), then per
+ * tensor A_m = M V^-1 with V the Hadamard product of the other Grams
+ * (Cholesky in FP64), column normalisation into lambda (2-norm on the first
+ * sweep ever, max(1, max |a|) after), and the new Gram. fits (optional,
+ * ntensors entries) receives 1 - ||X - model|| / ||X|| after the last sweep.
+ * stream NULL with use_graphs: a stream owned by the handle is used. */
+int tentaco_cpd_iterate(tentaco_cpd* c, int sweeps, int use_graphs, void* stream,
+                        double* fits);
+/* Copy factor m of tensor t (dims x R) and its lambda (R) to the host. */
+int tentaco_cpd_factor(const tentaco_cpd* c, int t, int m, float* factor, float* lambda);
+/* Replace factor m of tensor t (its Gram is recomputed). */
+int tentaco_cpd_set_factor(tentaco_cpd* c, int t, int m, const float* factor);
+int tentaco_cpd_free(tentaco_cpd* c);
 
 /* ---- learned variant selector ------------------------------------------ */
 /* No reference counterpart: it replaces running the exhaustive sweep
@@ -314,6 +341,10 @@ int tentaco_mlp_select(tentaco_mlp* m, int order, int R, const float* features,
  * that it runs tentaco_csf_autotune (source SWEEP). Under stream capture the
  * structural heuristic is used instead, since both would synchronise. */
 int tentaco_csf_set_selector(tentaco_csf* h, tentaco_mlp* m, float min_margin);
+/* Fallback when the margin is under min_margin: 0 (the default) runs the full
+ * sweep; k > 0 times only the k best-predicted variants that fit and records
+ * the fastest (source SELECTOR_TIMED), a fraction of the sweep's cost. */
+int tentaco_csf_set_selector_fallback(tentaco_csf* h, int top_k);
 
 /* TTV on device buffers: c_dev has the last mode's extent, A_dev has one
  * float per level N-2 node (nodes[N-2]). */

@@ -25,7 +25,7 @@ LIB_PATH = os.path.join(_HERE, "libtentaco.so")
 VARIANT_AUTO = -1
 VARIANT_DEFAULT = -2
 NUM_FEATURES = 24
-TUNED_NONE, TUNED_SWEEP, TUNED_SELECTOR, TUNED_SET = 0, 1, 2, 3
+TUNED_NONE, TUNED_SWEEP, TUNED_SELECTOR, TUNED_SET, TUNED_SELECTOR_TIMED = 0, 1, 2, 3, 4
 
 OK, EINVAL, EHIP, ENOMEM, EUNSUPPORTED, ENOGPU = 0, 1, 2, 3, 4, 5
 
@@ -114,6 +114,14 @@ def lib() -> ctypes.CDLL:
         "tentaco_csf_tuned": (c_int, [c_void_p, c_int, POINTER(c_int), POINTER(c_int)]),
         "tentaco_csf_features": (c_int, [c_void_p, c_int, POINTER(c_float)]),
         "tentaco_csf_set_selector": (c_int, [c_void_p, c_void_p, c_float]),
+        "tentaco_csf_set_selector_fallback": (c_int, [c_void_p, c_int]),
+        "tentaco_cpd_create": (c_int, [POINTER(c_void_p), POINTER(c_int), c_int, c_int, ctypes.c_uint64,
+                                       POINTER(c_void_p)]),
+        "tentaco_cpd_tune": (c_int, [c_void_p, c_int]),
+        "tentaco_cpd_iterate": (c_int, [c_void_p, c_int, c_int, c_void_p, POINTER(ctypes.c_double)]),
+        "tentaco_cpd_factor": (c_int, [c_void_p, c_int, c_int, POINTER(c_float), POINTER(c_float)]),
+        "tentaco_cpd_set_factor": (c_int, [c_void_p, c_int, c_int, POINTER(c_float)]),
+        "tentaco_cpd_free": (c_int, [c_void_p]),
         "tentaco_mlp_num_params": (c_int64, [c_int]),
         "tentaco_mlp_create": (c_int, [c_int, ctypes.c_uint64, POINTER(c_void_p)]),
         "tentaco_mlp_free": (c_int, [c_void_p]),
@@ -517,6 +525,10 @@ class DeviceCsf:
         self._selector = selector  # keep it alive while attached
         _check(lib().tentaco_csf_set_selector(self.h, selector.h if selector else None, min_margin))
 
+    def set_selector_fallback(self, top_k: int):
+        """0: full sweep when the margin is small; k > 0: time the top k picks."""
+        _check(lib().tentaco_csf_set_selector_fallback(self.h, top_k))
+
     def ttv(self, c: DeviceArray, y: DeviceArray, stream=None):
         _check(lib().tentaco_ttv_dev(self.h, c.ptr, y.ptr, stream))
 
@@ -740,6 +752,72 @@ class Selector:
     def close(self):
         if self.h:
             lib().tentaco_mlp_free(self.h)
+            self.h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Cpd:
+    """CP-ALS on the device for several tensors at once (tentaco_cpd_*).
+
+    mode_handles[t][m] is tensor t's DeviceCsf with mode m at its root;
+    level_modes[t][m] lists the tensor mode at each level of that CSF
+    (default: m first, then the other modes in increasing order, as
+    Csf.mode_csfs() builds them)."""
+
+    def __init__(self, mode_handles: Sequence[Sequence["DeviceCsf"]], R: int, seed: int = 0,
+                 level_modes=None):
+        nt = len(mode_handles)
+        hs = (c_void_p * (5 * nt))()
+        lm = (c_int * (25 * nt))()
+        self.orders = []
+        self.dims = []
+        for t, modes in enumerate(mode_handles):
+            N = len(modes)
+            self.orders.append(N)
+            for m, H in enumerate(modes):
+                hs[5 * t + m] = H.h
+                levels = (level_modes[t][m] if level_modes is not None
+                          else [m] + [k for k in range(N) if k != m])
+                for l, md in enumerate(levels):
+                    lm[(5 * t + m) * 5 + l] = md
+            dims = [0] * N
+            levels0 = level_modes[t][0] if level_modes is not None else list(range(N))
+            for l, md in enumerate(levels0):
+                dims[md] = modes[0].dims[l]
+            self.dims.append(dims)
+        self.R = R
+        self._keep = [list(m) for m in mode_handles]
+        self.h = c_void_p()
+        _check(lib().tentaco_cpd_create(hs, lm, nt, R, seed, ctypes.byref(self.h)))
+
+    def tune(self, reps: int = 3):
+        _check(lib().tentaco_cpd_tune(self.h, reps))
+
+    def iterate(self, sweeps: int = 1, use_graphs: bool = False, stream=None) -> np.ndarray:
+        """Run ALS sweeps; returns each tensor's fit after the last one."""
+        fits = (ctypes.c_double * len(self.orders))()
+        _check(lib().tentaco_cpd_iterate(self.h, sweeps, int(use_graphs), stream, fits))
+        return np.array(fits[:])
+
+    def factor(self, t: int, m: int):
+        a = np.zeros((self.dims[t][m], self.R), np.float32)
+        lam = np.zeros(self.R, np.float32)
+        _check(lib().tentaco_cpd_factor(self.h, t, m, _fp(a), _fp(lam)))
+        return a, lam
+
+    def set_factor(self, t: int, m: int, a):
+        a = np.ascontiguousarray(a, np.float32)
+        assert a.shape == (self.dims[t][m], self.R)
+        _check(lib().tentaco_cpd_set_factor(self.h, t, m, _fp(a)))
+
+    def close(self):
+        if self.h:
+            lib().tentaco_cpd_free(self.h)
             self.h = c_void_p()
 
     def __del__(self):

@@ -412,6 +412,10 @@ int tentaco_csf_reserve(tentaco_csf* h, int R, int variant) {
 // it for this rank), else run the sweep, which records the measured best.
 // Under stream capture neither may run (both synchronise), so the structural
 // heuristic answers instead.
+static int sweep_impl(tentaco_csf* h, int R, const float* const* factors_dev, float* A_dev,
+                      int32_t a_rows, int reps, float cut, void* stream, const bool* use,
+                      float* times_ms);
+
 static int auto_with_selector(tentaco_csf* h, int R, const float* const* factors_dev,
                               float* A_dev, int32_t a_rows, hipStream_t st, int* out) {
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
@@ -425,11 +429,32 @@ static int auto_with_selector(tentaco_csf* h, int R, const float* const* factors
   if (rc) return rc;
   int pick = -1;
   float margin = 0.f;
-  rc = mlp_select(h->selector, h->order, R, f, &pick, &margin, nullptr);
+  float pred[kNumVariants];
+  rc = mlp_select(h->selector, h->order, R, f, &pick, &margin, pred);
   if (rc) return rc;
   if (margin >= h->min_margin) {
     h->tune.set(R, pick, TENTACO_TUNED_SELECTOR);
     *out = pick;
+    return TENTACO_OK;
+  }
+  if (h->fallback_k > 0) {
+    // time only the k best-predicted variants that fit
+    int order_v[kNumVariants];
+    int n = 0;
+    for (int v = 0; v < kNumVariants; ++v)
+      if (variant_fits(h->order, R, v)) order_v[n++] = v;
+    std::stable_sort(order_v, order_v + n, [&](int a, int b) { return pred[a] < pred[b]; });
+    bool use[kNumVariants] = {};
+    for (int i = 0; i < std::min(n, h->fallback_k); ++i) use[order_v[i]] = true;
+    float t[kNumVariants];
+    rc = sweep_impl(h, R, factors_dev, A_dev, a_rows, 3, 0.f, st, use, t);
+    if (rc) return rc;
+    int best = -1;
+    for (int v = 0; v < kNumVariants; ++v)
+      if (t[v] >= 0.f && (best < 0 || t[v] < t[best])) best = v;
+    if (best < 0) best = pick;
+    h->tune.set(R, best, TENTACO_TUNED_SELECTOR_TIMED);
+    *out = best;
     return TENTACO_OK;
   }
   int best = -1;
@@ -470,11 +495,11 @@ int tentaco_mttkrp_dev(tentaco_csf* h, int R, const float* const* factors_dev, f
                        static_cast<hipStream_t>(stream));
 }
 
-int tentaco_csf_sweep(tentaco_csf* h, int R, const float* const* factors_dev, float* A_dev,
-                      int32_t a_rows, int reps, float cut, void* stream, float* times_ms) {
-  clear_error();
-  if (!h) return fail(TENTACO_EINVAL, "handle is NULL");
-  if (!times_ms) return fail(TENTACO_EINVAL, "times_ms is NULL");
+// The sweep over the variants with use[v] set (all when use is NULL);
+// times_ms[v] = -1 for the others.
+static int sweep_impl(tentaco_csf* h, int R, const float* const* factors_dev, float* A_dev,
+                      int32_t a_rows, int reps, float cut, void* stream, const bool* use,
+                      float* times_ms) {
   if (reps < 1) reps = 1;
   hipStream_t st = static_cast<hipStream_t>(stream);
   // reserve the largest tile's workspace first: no timed call allocates
@@ -514,6 +539,7 @@ int tentaco_csf_sweep(tentaco_csf* h, int R, const float* const* factors_dev, fl
   float best_probe = -1.f;
   for (int v = 0; v < kNumVariants; ++v) {
     times_ms[v] = -1.f;
+    if (use && !use[v]) continue;
     float p = 0.f;
     int rc = timed(v, 1, &p);
     if (rc == TENTACO_EUNSUPPORTED) {
@@ -549,6 +575,14 @@ int tentaco_csf_sweep(tentaco_csf* h, int R, const float* const* factors_dev, fl
   return TENTACO_OK;
 }
 
+int tentaco_csf_sweep(tentaco_csf* h, int R, const float* const* factors_dev, float* A_dev,
+                      int32_t a_rows, int reps, float cut, void* stream, float* times_ms) {
+  clear_error();
+  if (!h) return fail(TENTACO_EINVAL, "handle is NULL");
+  if (!times_ms) return fail(TENTACO_EINVAL, "times_ms is NULL");
+  return sweep_impl(h, R, factors_dev, A_dev, a_rows, reps, cut, stream, nullptr, times_ms);
+}
+
 int tentaco_csf_autotune(tentaco_csf* h, int R, const float* const* factors_dev, float* A_dev,
                          int32_t a_rows, int reps, void* stream, int* best, float* times_ms) {
   std::vector<float> t(static_cast<size_t>(kNumVariants), -1.f);
@@ -577,6 +611,13 @@ int tentaco_csf_tuned(const tentaco_csf* h, int R, int* variant, int* source) {
   const int v = h->tune.lookup(R, &src);
   if (variant) *variant = v;
   if (source) *source = v < 0 ? 0 : src;
+  return TENTACO_OK;
+}
+
+int tentaco_csf_set_selector_fallback(tentaco_csf* h, int top_k) {
+  if (!h) return fail(TENTACO_EINVAL, "handle is NULL");
+  if (top_k < 0) return fail(TENTACO_EINVAL, "top_k must be >= 0");
+  h->fallback_k = top_k;
   return TENTACO_OK;
 }
 

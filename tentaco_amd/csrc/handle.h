@@ -20,6 +20,7 @@ struct tentaco_csf {
   tt::CsfStats stats;
   tentaco_mlp* selector = nullptr;  // learned selector for AUTO (not owned)
   float min_margin = 0.05f;         // below it AUTO runs the sweep instead
+  int fallback_k = 0;               // > 0: time only the k best-predicted variants
 
   tt::CsfDev view() const {
     tt::CsfDev v;

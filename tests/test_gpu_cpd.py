@@ -1,0 +1,114 @@
+"""Device CP-ALS (batched MTTKRP + Hadamard/Cholesky solve + normalisation +
+Grams) against the FP64 CP-ALS oracle on the same initial factors."""
+
+import numpy as np
+import pytest
+
+import tentaco_amd as tt
+from oracle import cpd as ocpd
+
+pytestmark = pytest.mark.gpu
+
+
+def _tensor(rng, dims, density=0.6):
+    n = int(np.prod(dims) * density)
+    coords = np.stack([rng.integers(0, d, n) for d in dims]).astype(np.int32)
+    vals = rng.uniform(-1, 1, n).astype(np.float32)
+    B = tt.Csf.from_coo(dims, coords, vals)
+    # the dense tensor from the CSF's own (FP32-summed) values: walk every
+    # leaf's path of node indices from the root
+    Xc = np.zeros(dims, np.float32)
+    paths = [np.arange(B.nodes(0))]
+    for m in range(1, B.order):
+        pos = np.asarray(B.pos[m])
+        parent = np.repeat(np.arange(B.nodes(m - 1)), np.diff(pos))
+        paths = [p[parent] for p in paths] + [np.arange(B.nodes(m))]
+    coord = [np.asarray(B.crd[m])[paths[m]] for m in range(B.order)]
+    Xc[tuple(coord)] = np.asarray(B.vals)
+    return B, Xc.astype(np.float64)
+
+
+def _setup(seed, shapes, R):
+    rng = np.random.default_rng(seed)
+    hosts, dense, handles = [], [], []
+    for dims in shapes:
+        B, X = _tensor(rng, dims)
+        hosts.append(B)
+        dense.append(X)
+        handles.append([tt.DeviceCsf(Bm) for Bm in B.mode_csfs()])
+    c = tt.Cpd(handles, R, seed=seed)
+    inits = []
+    for t, dims in enumerate(shapes):
+        f = [rng.normal(size=(d, R)).astype(np.float32) for d in dims]
+        for m, a in enumerate(f):
+            c.set_factor(t, m, a)
+        inits.append(f)
+    return c, dense, inits, handles
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+SHAPES = [(12, 10, 8), (8, 7, 6, 5), (6, 5, 5, 4, 4)]
+
+
+@pytest.mark.parametrize("use_graphs", [False, True])
+def test_cpd_matches_oracle(gpu, use_graphs):
+    R = 16
+    c, dense, inits, _ = _setup(3, SHAPES, R)
+    fits1 = c.iterate(1, use_graphs=use_graphs)
+    for t, X in enumerate(dense):
+        A, lam, fits = ocpd.cp_als(X, inits[t], 1)
+        for m in range(X.ndim):
+            a, l = c.factor(t, m)
+            assert _rel(a, A[m]) <= 1e-4, (t, m, _rel(a, A[m]))
+        assert _rel(l, lam) <= 1e-4
+        assert fits1[t] == pytest.approx(fits[-1], abs=1e-4)
+    fits5 = c.iterate(4, use_graphs=use_graphs)
+    for t, X in enumerate(dense):
+        A, lam, fits = ocpd.cp_als(X, inits[t], 5)
+        for m in range(X.ndim):
+            a, _ = c.factor(t, m)
+            assert _rel(a, A[m]) <= 2e-3, (t, m, _rel(a, A[m]))
+        assert fits5[t] == pytest.approx(fits[-1], abs=1e-3)
+        assert fits5[t] >= fits1[t] - 1e-4  # ALS does not lose fit
+    c.close()
+
+
+def test_cpd_recovers_planted_tensor(gpu):
+    rng = np.random.default_rng(5)
+    R = 16
+    dims = (24, 20, 18)
+    fac = [rng.normal(size=(d, R)) for d in dims]
+    X = ocpd.reconstruct(rng.uniform(1, 2, R), fac).astype(np.float32)
+    coords = np.stack(np.meshgrid(*[np.arange(d) for d in dims], indexing="ij")).reshape(3, -1)
+    B = tt.Csf.from_coo(dims, coords.astype(np.int32), X.reshape(-1))
+    handles = [[tt.DeviceCsf(Bm) for Bm in B.mode_csfs()]]
+    c = tt.Cpd(handles, R, seed=1)
+    # an initialisation that leaves the ALS "swamp" within 20 sweeps (others,
+    # e.g. the next draws of rng, sit near a fit of 0.81 for over 100)
+    r2 = np.random.default_rng(1000)
+    init = [r2.normal(size=(d, R)).astype(np.float32) for d in dims]
+    for m, a in enumerate(init):
+        c.set_factor(0, m, a)
+    fits = [c.iterate(1)[0] for _ in range(40)]
+    _, _, ref = ocpd.cp_als(X.astype(np.float64), init, 40)
+    # the device follows the FP64 trajectory sweep by sweep
+    for k in (4, 9, 19):
+        assert fits[k] == pytest.approx(ref[k], abs=2e-3), (k, fits[k], ref[k])
+    # near a fit of 1, ||X||^2 + ||model||^2 - 2 <X, model> cancels: FP32 Grams
+    # leave ~1e-6 relative noise in the residual, ~1e-3 in the fit
+    assert all(b >= a - 5e-3 for a, b in zip(fits, fits[1:])), fits
+    assert fits[-1] > 0.99, fits[-5:]
+    c.close()
+
+
+def test_cpd_rejects_bad_input(gpu):
+    rng = np.random.default_rng(6)
+    B, _ = _tensor(rng, (6, 5, 4))
+    hs = [tt.DeviceCsf(Bm) for Bm in B.mode_csfs()]
+    with pytest.raises(tt.TentacoError):
+        tt.Cpd([hs], 24)  # R outside {16, 32, 64}
+    with pytest.raises(tt.TentacoError):
+        tt.Cpd([hs], 16, level_modes=[[[1, 0, 2], [1, 0, 2], [2, 0, 1]]])  # mode 0 not at its root

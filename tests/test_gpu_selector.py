@@ -155,6 +155,16 @@ def test_auto_uses_selector_pick_or_sweep(gpu):
     assert src == tt.TUNED_SWEEP and 0 <= v2 < K
     assert oracle.rel_l2(A.numpy(), ref) <= REL_L2_TOL
     H.close()
+    # ... or only the selector's top 2 are timed: the pick is one of them
+    H = tt.DeviceCsf(B)
+    H.set_selector(sel, 10.0)
+    H.set_selector_fallback(2)
+    H.mttkrp(fd, A, R, tt.VARIANT_AUTO)
+    v3, src = H.tuned(R)
+    top2 = [int(i) for i in np.argsort(pred, kind="stable")[:2]]
+    assert src == tt.TUNED_SELECTOR_TIMED and v3 in top2, (v3, top2)
+    assert oracle.rel_l2(A.numpy(), ref) <= REL_L2_TOL
+    H.close()
     sel.close()
 
 
