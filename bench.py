@@ -373,6 +373,14 @@ def run_batch_workload(tt, metrics, oracle, K):
     ref = [o.numpy() for o in outs]
     per_ms, _ = time_steps(per_task, max(3, K // 5))
     per_ms /= max(3, K // 5)
+
+    def per_task_default():
+        for H, fd, o in zip(tasks, facs_d, outs):
+            H.mttkrp(fd, o, R, tt.VARIANT_DEFAULT)
+
+    per_task_default()
+    per_def_ms, _ = time_steps(per_task_default, 3)
+    per_def_ms /= 3
     bt = tt.Batch(tasks, R)
     bt.set_operands(facs_d, outs, rows)
     (fam, blk, grp), tune_ms = bt.autotune(reps=3)
@@ -410,6 +418,8 @@ def run_batch_workload(tt, metrics, oracle, K):
         "nnz_mode_visits": visits,
         "alg_bytes": balg,
         "per_task_launch_ms": round(per_ms, 4),
+        "per_task_default_variant_ms": round(per_def_ms, 4),
+        "speedup_graph_vs_per_task_default": round(per_def_ms / g_ms, 3),
         "batched_eager_ms": round(eager_ms, 4),
         "batched_graph_ms": round(g_ms, 4),
         "speedup_graph_vs_per_task": round(per_ms / g_ms, 3),
@@ -423,6 +433,28 @@ def run_batch_workload(tt, metrics, oracle, K):
     log(f"batch: per-task {per_ms:.3f} ms, batched graph {g_ms:.3f} ms "
         f"({res['speedup_graph_vs_per_task']}x)")
     bt.close()
+    # CP-ALS on the same 64 tensors: every factor of every tensor updated per
+    # sweep (one batched MTTKRP per mode index, then the R x R solves,
+    # normalisation and Grams on the device)
+    per_tensor, k = [], 0
+    for B in tensors:
+        per_tensor.append(tasks[k:k + B.order])
+        k += B.order
+    cp = tt.Cpd(per_tensor, R, seed=1005)
+    cp.tune(reps=2)
+    f0 = cp.iterate(1, use_graphs=True)
+    tt.synchronize()
+    sweeps = max(3, K // 5)
+    t0 = time.perf_counter()
+    f1 = cp.iterate(sweeps, use_graphs=True)
+    sweep_ms = (time.perf_counter() - t0) * 1e3 / sweeps
+    res["cp_als"] = {"tensors": len(tensors), "R": R, "ms_per_sweep": round(sweep_ms, 3),
+                     "sweeps_timed": sweeps, "mean_fit_after_1": round(float(np.mean(f0)), 4),
+                     f"mean_fit_after_{sweeps + 1}": round(float(np.mean(f1)), 4),
+                     "fit_nondecreasing": bool(np.all(f1 >= f0 - 1e-3))}
+    log(f"CP-ALS: {sweep_ms:.2f} ms per sweep over {len(tensors)} tensors, mean fit "
+        f"{np.mean(f0):.4f} -> {np.mean(f1):.4f}")
+    cp.close()
     for H in tasks:
         H.close()
     return res

@@ -281,12 +281,27 @@ class Csf:
         del keep
         return _wrap_host_csf(h)
 
-    def mode_csfs(self) -> List["Csf"]:
-        """One CSF per mode ordering: mode m at the root, the others after it
-        in their original order."""
+    def mode_levels(self, m: int, policy: str = "natural") -> List[int]:
+        """Level order of the mode-m CSF: m at the root, then the other modes
+        in mode order ("natural"), by ascending dimension ("small_first", the
+        splatt-like order, largest mode at the leaves) or by descending
+        dimension ("large_first": the smallest mode, whose factor rows are
+        gathered once per nonzero, at the leaves)."""
+        rest = [k for k in range(self.order) if k != m]
+        if policy == "small_first":
+            rest.sort(key=lambda k: (self.dims[k], k))
+        elif policy == "large_first":
+            rest.sort(key=lambda k: (-self.dims[k], k))
+        elif policy != "natural":
+            raise ValueError(f"unknown mode order policy {policy!r}")
+        return [m] + rest
+
+    def mode_csfs(self, policy: str = "natural") -> List["Csf"]:
+        """One CSF per mode, mode m at the root (levels as mode_levels)."""
         out = []
         for m in range(self.order):
-            out.append(self if m == 0 else self.reorder([m] + [k for k in range(self.order) if k != m]))
+            lv = self.mode_levels(m, policy)
+            out.append(self if lv == list(range(self.order)) else self.reorder(lv))
         return out
 
     @classmethod

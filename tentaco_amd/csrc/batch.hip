@@ -12,8 +12,9 @@
 struct tentaco_batch {
   int ntasks = 0;
   int R = 0;
-  std::vector<tentaco_csf*> tasks;
-  std::vector<tt::BatchTaskDev> host_tasks;
+  std::vector<tentaco_csf*> tasks;  // sorted by order (stable)
+  std::vector<int> perm;            // sorted position -> the caller's task index
+  std::vector<tt::BatchTaskDev> host_tasks;  // sorted positions
   bool have_operands = false;
   // device tables
   tt::BatchTaskDev* d_tasks = nullptr;
@@ -109,14 +110,27 @@ int configure(tentaco_batch* b, int family, int block, int group) {
   L.group = group;
   L.R = b->R;
   L.NB = int32_t(NB);
-  int per_cu = 1;
-  rc = batch_occupancy(family, block, b->R, int32_t(NB), &per_cu);
-  if (rc) return rc;
   int dev = 0;
   TT_HIP_CHECK(hipGetDevice(&dev));
   hipDeviceProp_t prop;
   TT_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
-  L.grid = std::max(1, per_cu) * prop.multiProcessorCount;
+  // order groups of the sorted task table and a persistent grid per order
+  L.grid = 0;
+  for (int N = 0; N <= kMaxOrder + 1; ++N) {
+    int s = 0;
+    while (s < b->ntasks && b->tasks[s]->order < N) ++s;
+    L.grp_task[N] = s;
+    L.grp_item[N] = int32_t(item[s]);
+  }
+  for (int N = 2; N <= kMaxOrder; ++N) {
+    L.grp_grid[N] = 0;
+    if (L.grp_item[N + 1] <= L.grp_item[N]) continue;
+    int per_cu = 1;
+    rc = batch_occupancy(family, block, b->R, int32_t(NB), N, &per_cu);
+    if (rc) return rc;
+    L.grp_grid[N] = std::max(1, per_cu) * prop.multiProcessorCount;
+    L.grid = std::max(L.grid, L.grp_grid[N]);
+  }
   return TENTACO_OK;
 }
 
@@ -138,9 +152,13 @@ int tentaco_batch_create(tentaco_csf* const* tasks, int ntasks, int R, tentaco_b
   auto* b = new tentaco_batch;
   b->ntasks = ntasks;
   b->R = R;
-  b->tasks.assign(tasks, tasks + ntasks);
+  b->perm.resize(size_t(ntasks));
+  for (int t = 0; t < ntasks; ++t) b->perm[size_t(t)] = t;
+  std::stable_sort(b->perm.begin(), b->perm.end(),
+                   [&](int x, int y) { return tasks[x]->order < tasks[y]->order; });
+  for (int t = 0; t < ntasks; ++t) b->tasks.push_back(tasks[b->perm[size_t(t)]]);
   b->host_tasks.resize(size_t(ntasks));
-  if (hipMalloc(&b->d_counter, sizeof(int32_t)) != hipSuccess) {
+  if (hipMalloc(&b->d_counter, sizeof(int32_t) * (kMaxOrder + 1)) != hipSuccess) {
     delete b;
     return fail(TENTACO_ENOMEM, "counter allocation failed");
   }
@@ -153,15 +171,16 @@ int tentaco_batch_set_operands(tentaco_batch* b, const float* const* factors_dev
   clear_error();
   if (!b || !factors_dev || !outs_dev || !out_rows) return fail(TENTACO_EINVAL, "NULL argument");
   std::vector<int64_t> fill(b->ntasks + 1, 0);
-  for (int t = 0; t < b->ntasks; ++t) {
-    const tentaco_csf* h = b->tasks[t];
+  for (int s = 0; s < b->ntasks; ++s) {
+    const int t = b->perm[size_t(s)];  // the caller's index of sorted position s
+    const tentaco_csf* h = b->tasks[s];
     if (out_rows[t] <= h->max_crd[0])
       return fail(TENTACO_EINVAL, "task %d output has %d rows, root coordinates reach %d", t,
                   out_rows[t], h->max_crd[0]);
     if (!outs_dev[t]) return fail(TENTACO_EINVAL, "task %d output is NULL", t);
     if (reinterpret_cast<uintptr_t>(outs_dev[t]) & 15)
       return fail(TENTACO_EINVAL, "task %d output is not 16-byte aligned", t);
-    BatchTaskDev& T = b->host_tasks[t];
+    BatchTaskDev& T = b->host_tasks[size_t(s)];
     T.B = h->view();
     for (int m = 1; m < h->order; ++m) {
       const float* f = factors_dev[size_t(t) * (kMaxOrder - 1) + (m - 1)];
@@ -169,7 +188,7 @@ int tentaco_batch_set_operands(tentaco_batch* b, const float* const* factors_dev
       T.F.M[m] = f;
     }
     T.A = outs_dev[t];
-    fill[t + 1] = fill[t] + int64_t(out_rows[t]) * b->R;
+    fill[s + 1] = fill[s] + int64_t(out_rows[t]) * b->R;
   }
   (void)hipFree(b->d_fill_off);
   b->d_fill_off = nullptr;

@@ -93,10 +93,17 @@ struct BatchLaunch {
   int schedule = 0;  // 0: device work counter, 1: static cyclic items
   int32_t NB = 0;
   int grid = 0;
+  // the device task table is sorted by order: tasks [grp_task[N],
+  // grp_task[N+1]) of order N hold items [grp_item[N], grp_item[N+1]) and run
+  // in their own persistent launch of grp_grid[N] workgroups; counter holds
+  // kMaxOrder + 1 work counters, one per order
+  int32_t grp_task[kMaxOrder + 2] = {};
+  int32_t grp_item[kMaxOrder + 2] = {};
+  int grp_grid[kMaxOrder + 1] = {};
 };
 
 int launch_batch(const BatchLaunch& L, hipStream_t stream);
-int batch_occupancy(int family, int block, int R, int32_t NB, int* per_cu);
+int batch_occupancy(int family, int block, int R, int32_t NB, int order, int* per_cu);
 
 // Fill n floats with value on `stream` (kernel, not hipMemsetAsync, so it is
 // safe inside captured graphs).

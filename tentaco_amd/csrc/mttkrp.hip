@@ -655,7 +655,7 @@ __global__ __launch_bounds__(256) void batch_prep_kernel(const BatchTaskDev* __r
                                                          int32_t NB) {
   const int64_t tid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t nthr = int64_t(gridDim.x) * blockDim.x;
-  if (tid == 0) *counter = 0;
+  if (tid <= kMaxOrder) counter[tid] = 0;  // one work counter per order
   // Zero the outputs as float4 quads (every output holds rows x R floats
   // with R a multiple of 4 and starts 16-byte aligned, which the host
   // checks). Each workgroup takes one contiguous range of quads, finds the
@@ -699,32 +699,35 @@ __global__ __launch_bounds__(256) void batch_prep_kernel(const BatchTaskDev* __r
   }
 }
 
-template <int R, bool EP>
+// Persistent workgroups over the items of one order's tasks (the device
+// task table is sorted by order: tasks [t_lo, t_hi) hold items [i_lo, i_hi)),
+// so each order runs with its own register and LDS footprint. schedule 0
+// pulls items from the order's device counter (reset by the prep launch) in
+// arrival order; schedule 1 is a static cyclic assignment (item = i_lo +
+// block + k * grid). Either way a workgroup's items increase, so its task
+// cursor only moves forward.
+template <int N, int R, bool EP>
 __global__ __launch_bounds__(512) void batch_kernel(const BatchTaskDev* __restrict__ tasks,
-                                                    int32_t ntasks,
+                                                    int32_t t_lo, int32_t t_hi,
                                                     const int64_t* __restrict__ item_off,
-                                                    int32_t total_items,
+                                                    int32_t i_lo, int32_t i_hi,
                                                     int32_t* __restrict__ counter, int Rr,
                                                     int32_t G, int32_t NB, int dynamic) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int32_t s_item;
-  // schedule 0: items are pulled from the device counter (reset by the prep
-  // launch) in arrival order; schedule 1: static cyclic assignment (item =
-  // block + k * grid), no shared counter. Either way a workgroup's items
-  // increase, so its task cursor only moves forward.
   int32_t t = -1;
   int64_t t_end = 0;
-  int32_t item = blockIdx.x;
+  int32_t item = i_lo + int32_t(blockIdx.x);
   for (;;) {
     __syncthreads();  // the previous item's LDS reads are done
     if (dynamic) {
-      if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+      if (threadIdx.x == 0) s_item = i_lo + atomicAdd(counter, 1);
       __syncthreads();
       item = __builtin_amdgcn_readfirstlane(s_item);
     }
-    if (item >= total_items) break;
+    if (item >= i_hi) break;
     if (t < 0) {
-      t = upper_index(item_off, ntasks, item);
+      t = t_lo + upper_index(item_off + t_lo, t_hi - t_lo, item);
       t_end = item_off[t + 1];
     }
     while (item >= t_end) {
@@ -733,22 +736,34 @@ __global__ __launch_bounds__(512) void batch_kernel(const BatchTaskDev* __restri
     }
     const BatchTaskDev& T = tasks[t];
     const int32_t b = item - int32_t(item_off[t]);
-    switch (T.B.order) {
-      case 2: edge_block<2, R, EP>(T.B, T.F, T.A, Rr, T.brk, b, G, NB, smem); break;
-      case 3: edge_block<3, R, EP>(T.B, T.F, T.A, Rr, T.brk, b, G, NB, smem); break;
-      case 4: edge_block<4, R, EP>(T.B, T.F, T.A, Rr, T.brk, b, G, NB, smem); break;
-      default: edge_block<5, R, EP>(T.B, T.F, T.A, Rr, T.brk, b, G, NB, smem); break;
-    }
-    if (!dynamic) item += gridDim.x;
+    edge_block<N, R, EP>(T.B, T.F, T.A, Rr, T.brk, b, G, NB, smem);
+    if (!dynamic) item += int32_t(gridDim.x);
   }
+}
+
+template <int N, int R>
+int launch_batch_order(const BatchLaunch& L, hipStream_t st) {
+  const int32_t i_lo = L.grp_item[N], i_hi = L.grp_item[N + 1];
+  if (i_hi <= i_lo) return TENTACO_OK;
+  const int lpr = Row<R>::LPR;
+  size_t lds = edge_stage_ints(N, L.NB) * sizeof(int32_t);
+  if (L.family == kEdgeParallel) lds += edge_ep_bytes(L.block, lpr, L.R);
+  if (lds > 65536) return fail(TENTACO_EUNSUPPORTED, "batch LDS %zu B", lds);
+  const int grid = std::max(1, std::min<int>(L.grp_grid[N], i_hi - i_lo));
+  if (L.family == kEdgeParallel)
+    batch_kernel<N, R, true><<<grid, L.block, lds, st>>>(
+        L.tasks, L.grp_task[N], L.grp_task[N + 1], L.item_off, i_lo, i_hi, L.counter + N, L.R,
+        L.group, L.NB, L.schedule == 0);
+  else
+    batch_kernel<N, R, false><<<grid, L.block, lds, st>>>(
+        L.tasks, L.grp_task[N], L.grp_task[N + 1], L.item_off, i_lo, i_hi, L.counter + N, L.R,
+        L.group, L.NB, L.schedule == 0);
+  TT_HIP_CHECK(hipGetLastError());
+  return TENTACO_OK;
 }
 
 template <int R>
 int launch_batch_typed(const BatchLaunch& L, hipStream_t st) {
-  const int lpr = Row<R>::LPR;
-  size_t lds = edge_stage_ints(kMaxOrder, L.NB) * sizeof(int32_t);
-  if (L.family == kEdgeParallel) lds += edge_ep_bytes(L.block, lpr, L.R);
-  if (lds > 65536) return fail(TENTACO_EUNSUPPORTED, "batch LDS %zu B", lds);
   {
     const int64_t work = std::max<int64_t>(L.total_fill, L.total_bnd);
     batch_prep_kernel<<<grid_for(work, 256, 16384), 256, 0, st>>>(
@@ -756,34 +771,39 @@ int launch_batch_typed(const BatchLaunch& L, hipStream_t st) {
     TT_HIP_CHECK(hipGetLastError());
   }
   if (L.total_items == 0) return TENTACO_OK;
-  const int grid = std::max(1, std::min<int>(L.grid, L.total_items));
   ktimer_begin(st);
-  if (L.family == kEdgeParallel)
-    batch_kernel<R, true><<<grid, L.block, lds, st>>>(L.tasks, L.ntasks, L.item_off,
-                                                       L.total_items, L.counter, L.R, L.group, L.NB,
-                                                       L.schedule == 0);
-  else
-    batch_kernel<R, false><<<grid, L.block, lds, st>>>(L.tasks, L.ntasks, L.item_off,
-                                                        L.total_items, L.counter, L.R, L.group, L.NB,
-                                                        L.schedule == 0);
-  TT_HIP_CHECK(hipGetLastError());
+  int rc = launch_batch_order<2, R>(L, st);
+  if (!rc) rc = launch_batch_order<3, R>(L, st);
+  if (!rc) rc = launch_batch_order<4, R>(L, st);
+  if (!rc) rc = launch_batch_order<5, R>(L, st);
+  if (rc) return rc;
   ktimer_end(st, "batch_kernel");
   return TENTACO_OK;
 }
 
-template <int R>
-int batch_occupancy_typed(int family, int block, int Rr, int32_t NB, int* per_cu) {
+template <int N, int R>
+int batch_occupancy_order(int family, int block, int Rr, int32_t NB, int* per_cu) {
   const int lpr = Row<R>::LPR;
   hipError_t e;
-  size_t lds = edge_stage_ints(kMaxOrder, NB) * sizeof(int32_t);
+  size_t lds = edge_stage_ints(N, NB) * sizeof(int32_t);
   if (family == kEdgeParallel) {
     lds += edge_ep_bytes(block, lpr, Rr);
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, batch_kernel<R, true>, block, lds);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, batch_kernel<N, R, true>, block, lds);
   } else {
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, batch_kernel<R, false>, block, lds);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, batch_kernel<N, R, false>, block, lds);
   }
   if (e != hipSuccess) return fail(TENTACO_EHIP, "occupancy query: %s", hipGetErrorString(e));
   return TENTACO_OK;
+}
+
+template <int R>
+int batch_occupancy_typed(int family, int block, int Rr, int32_t NB, int order, int* per_cu) {
+  switch (order) {
+    case 2: return batch_occupancy_order<2, R>(family, block, Rr, NB, per_cu);
+    case 3: return batch_occupancy_order<3, R>(family, block, Rr, NB, per_cu);
+    case 4: return batch_occupancy_order<4, R>(family, block, Rr, NB, per_cu);
+    default: return batch_occupancy_order<5, R>(family, block, Rr, NB, per_cu);
+  }
 }
 
 int grid_for(int64_t work, int block, int cap) {
@@ -875,11 +895,11 @@ int launch_batch(const BatchLaunch& L, hipStream_t st) {
   }
 }
 
-int batch_occupancy(int family, int block, int R, int32_t NB, int* per_cu) {
+int batch_occupancy(int family, int block, int R, int32_t NB, int order, int* per_cu) {
   switch (R) {
-    case 16: return batch_occupancy_typed<16>(family, block, R, NB, per_cu);
-    case 32: return batch_occupancy_typed<32>(family, block, R, NB, per_cu);
-    case 64: return batch_occupancy_typed<64>(family, block, R, NB, per_cu);
+    case 16: return batch_occupancy_typed<16>(family, block, R, NB, order, per_cu);
+    case 32: return batch_occupancy_typed<32>(family, block, R, NB, order, per_cu);
+    case 64: return batch_occupancy_typed<64>(family, block, R, NB, order, per_cu);
     default: return fail(TENTACO_EUNSUPPORTED, "batched launch supports R in {16,32,64}");
   }
 }
