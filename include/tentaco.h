@@ -346,6 +346,20 @@ int tentaco_csf_set_selector(tentaco_csf* h, tentaco_mlp* m, float min_margin);
  * the fastest (source SELECTOR_TIMED), a fraction of the sweep's cost. */
 int tentaco_csf_set_selector_fallback(tentaco_csf* h, int top_k);
 
+/* MTTKRP for the mode held at CSF level `level` (1..order-1) of an uploaded
+ * CSF, without a CSF rooted at that mode (one CSF for every mode instead of
+ * one per mode): A(j,:) = sum over nonzeros whose level-`level` coordinate
+ * is j of val * prod_{l != level} factors_dev[l](coordinate at level l, :).
+ * factors_dev has order entries (the root level's factor included; the
+ * entry for `level` is not read). A_dev (a_rows x R, a_rows above the
+ * level's largest coordinate) is zero-filled, then accumulated with FP32
+ * atomics, so results are not bitwise reproducible across runs. No
+ * reference counterpart: the reference builds one CSF per mode through
+ * splatt (CSF_ONEMODE, ds.h:355-356). */
+int tentaco_mttkrp_level_dev(tentaco_csf* h, int level, int R,
+                             const float* const* factors_dev, float* A_dev,
+                             int32_t a_rows, void* stream);
+
 /* TTV on device buffers: c_dev has the last mode's extent, A_dev has one
  * float per level N-2 node (nodes[N-2]). */
 int tentaco_ttv_dev(tentaco_csf* h, const float* c_dev, float* A_dev,
@@ -394,6 +408,32 @@ int tentaco_batch_free(tentaco_batch* b);
  * both and leaves the faster set. */
 int tentaco_batch_set_schedule(tentaco_batch* b, int schedule);
 int tentaco_batch_get_schedule(const tentaco_batch* b, int* schedule);
+
+/* ---- stream-ordered host staging -------------------------------------- */
+/* The replacement for the per-call cudaMalloc / H2D / kernel / D2H / free
+ * round trip of copy_to_device_mttkrp / copy_from_device_mttkrp /
+ * free_tensors_mttkrp (gpu_library.h:780-945). Operand arrays in pinned host
+ * memory (tentaco_host_alloc) transfer by DMA without a staging copy and
+ * overlap with work on other streams. */
+int tentaco_host_alloc(void** ptr, size_t bytes);  /* hipHostMalloc, pinned */
+int tentaco_host_free(void* ptr);
+int tentaco_stream_create(void** stream);          /* default priority */
+int tentaco_stream_destroy(void* stream);
+int tentaco_stream_synchronize(void* stream);
+/* tentaco_csf_upload with the copies enqueued on `stream` (validation runs on
+ * the host first). B's arrays must stay valid until the stream has passed
+ * the copies. */
+int tentaco_csf_upload_async(const taco_tensor_t* B, int order, void* stream,
+                             tentaco_csf** out);
+/* MTTKRP from host operands into host A on a device-resident CSF: the
+ * factors (levels 1..order-1, dimensions {rows, R}) are copied into device
+ * buffers kept by the handle, the kernels run, and A is copied back, all
+ * enqueued on `stream`; the call returns before they finish. The host arrays
+ * must stay valid (and A unread) until the stream is synchronized. AUTO with
+ * a selector may run a sweep, which synchronizes. Calls on one handle share
+ * its buffers: order them on one stream. */
+int tentaco_mttkrp_host_async(tentaco_csf* h, int R, taco_tensor_t* const* factors,
+                              taco_tensor_t* A, int variant, void* stream);
 
 /* Device memory helpers for hosts without their own HIP runtime binding. */
 int tentaco_dev_malloc(void** ptr, size_t bytes);

@@ -106,6 +106,8 @@ def lib() -> ctypes.CDLL:
         "tentaco_csf_reserve": (c_int, [c_void_p, c_int, c_int]),
         "tentaco_mttkrp_dev": (c_int, [c_void_p, c_int, POINTER(c_void_p), c_void_p, c_int32, c_int, c_void_p]),
         "tentaco_ttv_dev": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+        "tentaco_mttkrp_level_dev": (c_int, [c_void_p, c_int, c_int, POINTER(c_void_p), c_void_p, c_int32,
+                                             c_void_p]),
         "tentaco_csf_autotune": (c_int, [c_void_p, c_int, POINTER(c_void_p), c_void_p, c_int32, c_int,
                                          c_void_p, POINTER(c_int), POINTER(c_float)]),
         "tentaco_csf_set_tuned": (c_int, [c_void_p, c_int, c_int]),
@@ -150,6 +152,13 @@ def lib() -> ctypes.CDLL:
         "tentaco_batch_free": (c_int, [c_void_p]),
         "tentaco_batch_set_schedule": (c_int, [c_void_p, c_int]),
         "tentaco_batch_get_schedule": (c_int, [c_void_p, POINTER(c_int)]),
+        "tentaco_host_alloc": (c_int, [POINTER(c_void_p), c_size_t]),
+        "tentaco_host_free": (c_int, [c_void_p]),
+        "tentaco_stream_create": (c_int, [POINTER(c_void_p)]),
+        "tentaco_stream_destroy": (c_int, [c_void_p]),
+        "tentaco_stream_synchronize": (c_int, [c_void_p]),
+        "tentaco_csf_upload_async": (c_int, [P, c_int, c_void_p, POINTER(c_void_p)]),
+        "tentaco_mttkrp_host_async": (c_int, [c_void_p, c_int, POINTER(P), P, c_int, c_void_p]),
         "tentaco_dev_malloc": (c_int, [POINTER(c_void_p), c_size_t]),
         "tentaco_dev_free": (c_int, [c_void_p]),
         "tentaco_memcpy_h2d": (c_int, [c_void_p, c_void_p, c_size_t]),
@@ -522,6 +531,39 @@ class DeviceCsf:
         _check(lib().tentaco_csf_sweep(self.h, R, arr, A.ptr, rows, reps, cut, stream, times))
         return [float(t) for t in times]
 
+    @classmethod
+    def upload_async(cls, B: "Csf", stream: "Stream") -> "DeviceCsf":
+        """Upload with the copies enqueued on `stream`; B must stay alive
+        until the stream is synchronized."""
+        self = cls.__new__(cls)
+        t, keep = B.as_taco()
+        self._keep_async = (B, t, keep)
+        self.h = c_void_p()
+        _check(lib().tentaco_csf_upload_async(ctypes.byref(t), B.order, stream.h, ctypes.byref(self.h)))
+        self.order = B.order
+        self.dims = B.dims
+        self.nnz = B.nnz
+        self.nodes = [B.nodes(m) for m in range(B.order)]
+        return self
+
+    def mttkrp_host_async(self, factors: Sequence[np.ndarray], A: np.ndarray, R: int,
+                          stream: "Stream", variant: int = VARIANT_AUTO):
+        """Host factors (levels 1..order-1) -> host A, enqueued on `stream`;
+        keep the arrays alive and A unread until stream.synchronize()."""
+        tf = [dense_taco(f) for f in factors]
+        tA, kA = dense_taco(A)
+        arr = (POINTER(TacoTensor) * len(tf))(*[ctypes.pointer(t) for t, _ in tf])
+        self._keep_call = (tf, tA, kA, arr, factors, A)
+        _check(lib().tentaco_mttkrp_host_async(self.h, R, arr, ctypes.byref(tA), variant, stream.h))
+
+    def mttkrp_level(self, level: int, factors: Sequence[Optional[DeviceArray]], A: DeviceArray,
+                     R: int, stream=None, a_rows: Optional[int] = None):
+        """MTTKRP for the mode at CSF level `level` on this CSF; factors[l]
+        for every level l (the entry for `level` may be None)."""
+        arr = (c_void_p * len(factors))(*[None if f is None else f.ptr for f in factors])
+        rows = self.dims[level] if a_rows is None else a_rows
+        _check(lib().tentaco_mttkrp_level_dev(self.h, level, R, arr, A.ptr, rows, stream))
+
     def tuned(self, R: int):
         """(variant, source) recorded for rank R; source is one of TUNED_*."""
         v, src = c_int(), c_int()
@@ -840,3 +882,62 @@ class Cpd:
             self.close()
         except Exception:
             pass
+
+
+class Stream:
+    """A HIP stream at the default priority (tentaco_stream_create)."""
+
+    def __init__(self):
+        self.h = c_void_p()
+        _check(lib().tentaco_stream_create(ctypes.byref(self.h)))
+
+    def synchronize(self):
+        _check(lib().tentaco_stream_synchronize(self.h))
+
+    def close(self):
+        if self.h:
+            lib().tentaco_stream_destroy(self.h)
+            self.h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _PinnedOwner:
+    def __init__(self, ptr):
+        self.ptr = ptr
+
+    def __del__(self):
+        try:
+            lib().tentaco_host_free(self.ptr)
+        except Exception:
+            pass
+
+
+class PinnedArray(np.ndarray):
+    """numpy array over pinned host memory; views keep it (and the memory)
+    alive through their base."""
+
+
+def pinned_empty(shape, dtype=np.float32) -> np.ndarray:
+    """A numpy array in pinned (page-locked) host memory (tentaco_host_alloc);
+    the memory is freed when the array and its views are collected."""
+    dtype = np.dtype(dtype)
+    shape = tuple(np.atleast_1d(shape).tolist()) if not isinstance(shape, tuple) else shape
+    n = int(np.prod(shape)) if shape else 1
+    nbytes = max(n * dtype.itemsize, 4)
+    ptr = c_void_p()
+    _check(lib().tentaco_host_alloc(ctypes.byref(ptr), nbytes))
+    buf = (ctypes.c_char * nbytes).from_address(ptr.value)
+    arr = np.frombuffer(buf, dtype=dtype, count=n).reshape(shape).view(PinnedArray)
+    arr._owner = _PinnedOwner(ptr)
+    return arr
+
+
+def pinned_copy(a: np.ndarray) -> np.ndarray:
+    out = pinned_empty(a.shape, a.dtype)
+    out[...] = a
+    return out

@@ -157,7 +157,23 @@ void free_handle(tentaco_csf* h) {
   }
   (void)hipFree(h->d_vals);
   (void)hipFree(h->ws);
+  for (int m = 0; m < kMaxOrder; ++m) (void)hipFree(h->stage_f[m]);
+  (void)hipFree(h->stage_a);
   delete h;
+}
+
+// Grow a device buffer to at least `bytes` (contents are not kept).
+int grow(float** p, size_t* cap, size_t bytes) {
+  if (bytes <= *cap && *p) return TENTACO_OK;
+  (void)hipFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  if (hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(bytes, 4)) != hipSuccess) {
+    *p = nullptr;
+    return fail(TENTACO_ENOMEM, "hipMalloc of %zu bytes failed", bytes);
+  }
+  *cap = bytes;
+  return TENTACO_OK;
 }
 
 int reserve_ws(tentaco_csf* h, size_t bytes) {
@@ -377,6 +393,119 @@ int tentaco_csf_upload(const taco_tensor_t* B, int order, tentaco_csf** out) {
   return TENTACO_OK;
 }
 
+int tentaco_csf_upload_async(const taco_tensor_t* B, int order, void* stream, tentaco_csf** out) {
+  clear_error();
+  if (!out) return fail(TENTACO_EINVAL, "out is NULL");
+  *out = nullptr;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  tentaco_csf* h = new tentaco_csf;
+  h->order = order;
+  int rc = validate_csf(B, order, h);
+  if (rc) {
+    delete h;
+    return rc;
+  }
+  auto put = [&](int32_t** dst, const int32_t* src, int64_t n) -> int {
+    const size_t bytes = size_t(std::max<int64_t>(n, 1)) * sizeof(int32_t);
+    if (hipMalloc(reinterpret_cast<void**>(dst), bytes) != hipSuccess) {
+      *dst = nullptr;
+      return fail(TENTACO_ENOMEM, "hipMalloc of %zu bytes failed", bytes);
+    }
+    if (n > 0) TT_HIP_CHECK(hipMemcpyAsync(*dst, src, size_t(n) * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    return TENTACO_OK;
+  };
+  rc = put(&h->d_crd0, B->indices[0][1], h->nodes[0]);
+  for (int m = 1; m < order && !rc; ++m) {
+    rc = put(&h->d_pos[m], B->indices[m][0], int64_t(h->nodes[m - 1]) + 1);
+    if (!rc) rc = put(&h->d_crd[m], B->indices[m][1], h->nodes[m]);
+  }
+  if (!rc) {
+    const int64_t nnz = h->nodes[order - 1];
+    const size_t bytes = size_t(std::max<int64_t>(nnz, 1)) * sizeof(float);
+    if (hipMalloc(reinterpret_cast<void**>(&h->d_vals), bytes) != hipSuccess) {
+      h->d_vals = nullptr;
+      rc = fail(TENTACO_ENOMEM, "hipMalloc of %zu bytes failed", bytes);
+    } else if (nnz > 0 && hipMemcpyAsync(h->d_vals, B->vals, size_t(nnz) * sizeof(float),
+                                         hipMemcpyHostToDevice, st) != hipSuccess) {
+      rc = fail(TENTACO_EHIP, "value upload failed");
+    }
+  }
+  if (rc) {
+    (void)hipStreamSynchronize(st);
+    free_handle(h);
+    return rc;
+  }
+  *out = h;
+  return TENTACO_OK;
+}
+
+int tentaco_mttkrp_host_async(tentaco_csf* h, int R, taco_tensor_t* const* factors,
+                              taco_tensor_t* A, int variant, void* stream) {
+  clear_error();
+  if (!h) return fail(TENTACO_EINVAL, "handle is NULL");
+  if (!A || !A->dimensions || !A->vals) return fail(TENTACO_EINVAL, "A is NULL/incomplete");
+  if (!factors) return fail(TENTACO_EINVAL, "factors is NULL");
+  if (R <= 0 || A->dimensions[1] != R)
+    return fail(TENTACO_EINVAL, "A has %d columns, expected R=%d", A->dimensions[1], R);
+  const int32_t a_rows = A->dimensions[0];
+  if (a_rows <= h->max_crd[0])
+    return fail(TENTACO_EINVAL, "A has %d rows but root coordinates reach %d", a_rows,
+                h->max_crd[0]);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const float* fdev[kMaxOrder] = {};
+  for (int m = 1; m < h->order; ++m) {
+    int rc = check_factor(factors[m - 1], m, h, R);
+    if (rc) return rc;
+    const size_t bytes = size_t(factors[m - 1]->dimensions[0]) * size_t(R) * sizeof(float);
+    rc = grow(&h->stage_f[m], &h->stage_f_cap[m], bytes);
+    if (rc) return rc;
+    TT_HIP_CHECK(hipMemcpyAsync(h->stage_f[m], factors[m - 1]->vals, bytes, hipMemcpyHostToDevice, st));
+    fdev[m - 1] = h->stage_f[m];
+  }
+  const size_t abytes = size_t(a_rows) * size_t(R) * sizeof(float);
+  int rc = grow(&h->stage_a, &h->stage_a_cap, abytes);
+  if (rc) return rc;
+  rc = tentaco_mttkrp_dev(h, R, fdev, h->stage_a, a_rows, variant, stream);
+  if (rc) return rc;
+  TT_HIP_CHECK(hipMemcpyAsync(A->vals, h->stage_a, abytes, hipMemcpyDeviceToHost, st));
+  return TENTACO_OK;
+}
+
+int tentaco_host_alloc(void** ptr, size_t bytes) {
+  clear_error();
+  if (!ptr) return fail(TENTACO_EINVAL, "ptr is NULL");
+  *ptr = nullptr;
+  if (hipHostMalloc(ptr, std::max<size_t>(bytes, 4), hipHostMallocDefault) != hipSuccess) {
+    *ptr = nullptr;
+    return fail(TENTACO_ENOMEM, "hipHostMalloc of %zu bytes failed", bytes);
+  }
+  return TENTACO_OK;
+}
+
+int tentaco_host_free(void* ptr) {
+  if (ptr) TT_HIP_CHECK(hipHostFree(ptr));
+  return TENTACO_OK;
+}
+
+int tentaco_stream_create(void** stream) {
+  clear_error();
+  if (!stream) return fail(TENTACO_EINVAL, "stream is NULL");
+  hipStream_t s = nullptr;
+  TT_HIP_CHECK(hipStreamCreate(&s));  // default priority
+  *stream = s;
+  return TENTACO_OK;
+}
+
+int tentaco_stream_destroy(void* stream) {
+  if (stream) TT_HIP_CHECK(hipStreamDestroy(static_cast<hipStream_t>(stream)));
+  return TENTACO_OK;
+}
+
+int tentaco_stream_synchronize(void* stream) {
+  TT_HIP_CHECK(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+  return TENTACO_OK;
+}
+
 int tentaco_csf_free(tentaco_csf* h) {
   free_handle(h);
   return TENTACO_OK;
@@ -573,6 +702,27 @@ static int sweep_impl(tentaco_csf* h, int R, const float* const* factors_dev, fl
     times_ms[v] = t[t.size() / 2];
   }
   return TENTACO_OK;
+}
+
+int tentaco_mttkrp_level_dev(tentaco_csf* h, int level, int R, const float* const* factors_dev,
+                             float* A_dev, int32_t a_rows, void* stream) {
+  clear_error();
+  if (!h) return fail(TENTACO_EINVAL, "handle is NULL");
+  if (!A_dev || !factors_dev) return fail(TENTACO_EINVAL, "NULL device buffer");
+  if (R <= 0) return fail(TENTACO_EINVAL, "R=%d", R);
+  if (level < 1 || level >= h->order)
+    return fail(TENTACO_EINVAL, "level %d outside 1..%d", level, h->order - 1);
+  if (a_rows <= h->max_crd[level])
+    return fail(TENTACO_EINVAL, "A has %d rows but level-%d coordinates reach %d", a_rows, level,
+                h->max_crd[level]);
+  FactorsDev F;
+  for (int l = 0; l < h->order; ++l) {
+    if (l == level) continue;
+    if (!factors_dev[l]) return fail(TENTACO_EINVAL, "factor of level %d is NULL", l);
+    F.M[l] = factors_dev[l];
+  }
+  return launch_mttkrp_level(h->view(), F, level, R, A_dev, a_rows,
+                             static_cast<hipStream_t>(stream));
 }
 
 int tentaco_csf_sweep(tentaco_csf* h, int R, const float* const* factors_dev, float* A_dev,

@@ -18,6 +18,12 @@ struct tentaco_csf {
   size_t ws_bytes = 0;
   tt::TuneTable tune;  // per-rank best variant found by the autotuner
   tt::CsfStats stats;
+  // device buffers of the stream-ordered host-pointer path (factors of
+  // levels 1..order-1 and the output), grown on demand
+  float* stage_f[tt::kMaxOrder] = {};
+  size_t stage_f_cap[tt::kMaxOrder] = {};
+  float* stage_a = nullptr;
+  size_t stage_a_cap = 0;
   tentaco_mlp* selector = nullptr;  // learned selector for AUTO (not owned)
   float min_margin = 0.05f;         // below it AUTO runs the sweep instead
   int fallback_k = 0;               // > 0: time only the k best-predicted variants

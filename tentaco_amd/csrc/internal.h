@@ -66,6 +66,12 @@ size_t ttv_workspace_bytes(const CsfDev& B);
 int launch_mttkrp(const CsfDev& B, const FactorsDev& F, int R, float* A,
                   int32_t a_rows, bool dense_root, const Variant& vr,
                   void* ws, hipStream_t stream);
+// Enqueue the MTTKRP of the mode held at CSF level `level` (1..order-1) on
+// the same CSF: F.M[l] is the factor of level l for every l != level
+// (including the root, l = 0); A (a_rows x R) is zero-filled, then
+// accumulated atomically.
+int launch_mttkrp_level(const CsfDev& B, const FactorsDev& F, int level, int R, float* A,
+                        int32_t a_rows, hipStream_t stream);
 // Enqueue one TTV: y has nodes[order-2] floats, c the last mode's extent.
 int launch_ttv(const CsfDev& B, const float* c, float* y, void* ws,
                hipStream_t stream);

@@ -444,6 +444,63 @@ __device__ __forceinline__ void subtree(const CsfDev& B, const FactorsDev& F,
   }
 }
 
+// MTTKRP for the mode held at level L >= 1 of the CSF, without a CSF rooted
+// at that mode:
+//   A_L(j,:) = sum over nonzeros with level-L coordinate j of
+//              val * prod_{l != L} M_l(coordinate at level l, :),
+// with M_0 the root mode's factor. One row group per level-(L-1) node p: the
+// product of the factor rows of p and its ancestors (found by upper-bound
+// searches up the pos arrays) times, for every child y of p, the factorised
+// sum of y's subtree below level L, is added atomically into row crd_L[y]
+// of the zero-filled A (children of different parents can share a
+// coordinate).
+template <int N, int L, int R>
+__global__ __launch_bounds__(256) void level_kernel(CsfDev B, FactorsDev F, float* __restrict__ A,
+                                                    int Rr) {
+  using RO = Row<R>;
+  using V = typename RO::V;
+  constexpr int NV = RO::NV;
+  constexpr int LPR = RO::LPR;
+  const int c = int(threadIdx.x) % LPR;
+  const int64_t g0 = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / LPR;
+  const int64_t ng = int64_t(gridDim.x) * blockDim.x / LPR;
+  const int32_t np = B.nodes[L - 1];
+  for (int64_t p = g0; p < np; p += ng) {
+    V above[NV];
+    {
+      const int32_t crd = (L - 1 == 0) ? B.crd0[p] : B.crd[L - 1][p];
+      RO::load(above, F.M[L - 1], crd, c, Rr);
+    }
+    int32_t a = int32_t(p);
+#pragma unroll
+    for (int l = L - 2; l >= 0; --l) {
+      a = search_le(B.pos[l + 1], 0, B.nodes[l] - 1, a);
+      V r[NV];
+      RO::load(r, F.M[l], l == 0 ? B.crd0[a] : B.crd[l][a], c, Rr);
+      RO::hmul(above, r);
+    }
+    const int32_t y0 = B.pos[L][p], y1 = B.pos[L][p + 1];
+    for (int32_t y = y0; y < y1; ++y) {
+      V out[NV];
+      if constexpr (L == N - 1) {
+        RO::zero(out);
+        RO::axpy(out, B.vals[y], above);
+      } else {
+        V below[NV];
+        RO::zero(below);
+        for (int32_t ch = B.pos[L + 1][y]; ch < B.pos[L + 1][y + 1]; ++ch) {
+          V s2[NV];
+          subtree<N, L + 1, R>(B, F, ch, c, Rr, s2);
+          RO::add(below, s2);
+        }
+        RO::zero(out);
+        RO::hfma(out, above, below);
+      }
+      RO::atomic_add(A, B.crd[L][y], c, Rr, out);
+    }
+  }
+}
+
 // Order-3 slice: the wavefront walks the slice's level-1 fibers 64 at a time.
 // Lane L loads fiber (base+L)'s bounds, coordinate and first nonzero with
 // coalesced loads; row group g then takes fibers base + t*NG + g through
@@ -864,6 +921,54 @@ int launch_typed(const CsfDev& B, const FactorsDev& F, int Rr, float* A,
   return TENTACO_OK;
 }
 
+template <int N, int L, int R>
+int launch_level_typed(const CsfDev& B, const FactorsDev& F, int Rr, float* A, int32_t a_rows,
+                       hipStream_t st) {
+  const int64_t a_elems = int64_t(a_rows) * Rr;
+  fill_kernel<<<grid_for(a_elems / 4 + 1, 256, 16384), 256, 0, st>>>(A, a_elems, 0.f);
+  TT_HIP_CHECK(hipGetLastError());
+  const int32_t np = B.nodes[L - 1];
+  if (np <= 0) return TENTACO_OK;
+  const int64_t groups = np;
+  const int per_block = 256 / Row<R>::LPR;
+  const int grid = grid_for(groups, per_block, 65536);
+  ktimer_begin(st);
+  level_kernel<N, L, R><<<grid, 256, 0, st>>>(B, F, A, Rr);
+  TT_HIP_CHECK(hipGetLastError());
+  ktimer_end(st, "level_kernel");
+  return TENTACO_OK;
+}
+
+template <int N, int R>
+int launch_level_order(const CsfDev& B, const FactorsDev& F, int level, int Rr, float* A,
+                       int32_t a_rows, hipStream_t st) {
+  if constexpr (N >= 2) {
+    if (level == 1) return launch_level_typed<N, 1, R>(B, F, Rr, A, a_rows, st);
+  }
+  if constexpr (N >= 3) {
+    if (level == 2) return launch_level_typed<N, 2, R>(B, F, Rr, A, a_rows, st);
+  }
+  if constexpr (N >= 4) {
+    if (level == 3) return launch_level_typed<N, 3, R>(B, F, Rr, A, a_rows, st);
+  }
+  if constexpr (N >= 5) {
+    if (level == 4) return launch_level_typed<N, 4, R>(B, F, Rr, A, a_rows, st);
+  }
+  return fail(TENTACO_EINVAL, "level %d outside 1..%d", level, N - 1);
+}
+
+template <int R>
+int launch_level_rank(const CsfDev& B, const FactorsDev& F, int level, int Rr, float* A,
+                      int32_t a_rows, hipStream_t st) {
+  switch (B.order) {
+    case 2: return launch_level_order<2, R>(B, F, level, Rr, A, a_rows, st);
+    case 3: return launch_level_order<3, R>(B, F, level, Rr, A, a_rows, st);
+    case 4: return launch_level_order<4, R>(B, F, level, Rr, A, a_rows, st);
+    case 5: return launch_level_order<5, R>(B, F, level, Rr, A, a_rows, st);
+    default: return fail(TENTACO_EUNSUPPORTED, "order %d outside 2..5", B.order);
+  }
+}
+
 template <int N>
 int launch_order(const CsfDev& B, const FactorsDev& F, int R, float* A,
                  int32_t a_rows, bool dense_root, const Variant& vr, void* ws,
@@ -962,6 +1067,18 @@ size_t mttkrp_workspace_bytes(const CsfDev& B, int R, const Variant& vr) {
   const int64_t NB = nnz_per_block(vr, R);
   const int64_t nblk = (int64_t(B.nnz) + NB - 1) / NB;
   return size_t(nblk + 1) * size_t(B.order - 1) * sizeof(int32_t);
+}
+
+int launch_mttkrp_level(const CsfDev& B, const FactorsDev& F, int level, int R, float* A,
+                        int32_t a_rows, hipStream_t st) {
+  if (R <= 0 || R > Lay<0>::kMaxRank)
+    return fail(TENTACO_EUNSUPPORTED, "rank R=%d outside 1..%d", R, Lay<0>::kMaxRank);
+  switch (R) {
+    case 16: return launch_level_rank<16>(B, F, level, R, A, a_rows, st);
+    case 32: return launch_level_rank<32>(B, F, level, R, A, a_rows, st);
+    case 64: return launch_level_rank<64>(B, F, level, R, A, a_rows, st);
+    default: return launch_level_rank<0>(B, F, level, R, A, a_rows, st);
+  }
 }
 
 int launch_mttkrp(const CsfDev& B, const FactorsDev& F, int R, float* A,

@@ -1,0 +1,67 @@
+"""MTTKRP for a non-root mode on one CSF (no per-mode CSF), against the FP64
+dense oracle, every order and level."""
+
+import numpy as np
+import pytest
+
+import tentaco_amd as tt
+from helpers import REL_L2_TOL
+from oracle import cpd as ocpd
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(order, seed, n=3000):
+    rng = np.random.default_rng(seed)
+    dims = tuple(int(x) for x in rng.integers(4, 14, order))
+    coords = np.stack([rng.integers(0, d, n) for d in dims]).astype(np.int32)
+    vals = rng.uniform(-1, 1, n).astype(np.float32)
+    B = tt.Csf.from_coo(dims, coords, vals)
+    X = np.zeros(dims)
+    np.add.at(X, tuple(coords.astype(np.int64)), vals.astype(np.float64))
+    return rng, B, X
+
+
+@pytest.mark.parametrize("order", [2, 3, 4, 5])
+@pytest.mark.parametrize("R", [16, 32, 7])
+def test_level_mttkrp_matches_oracle(gpu, order, R):
+    rng, B, X = _case(order, 10 * order + R)
+    H = tt.DeviceCsf(B)
+    facs = [rng.uniform(-1, 1, (d, R)).astype(np.float32) for d in B.dims]
+    fd = [tt.DeviceArray.from_numpy(f) for f in facs]
+    for level in range(1, order):
+        ref = ocpd.mttkrp_mode(X, [f.astype(np.float64) for f in facs], level)
+        A = tt.DeviceArray.from_numpy(np.full((B.dims[level], R), 5.0, np.float32))  # poisoned
+        args = [None if l == level else fd[l] for l in range(order)]
+        H.mttkrp_level(level, args, A, R)
+        got = A.numpy()
+        e = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+        assert e <= REL_L2_TOL, (order, R, level, e)
+    H.close()
+
+
+def test_level_mttkrp_skewed_and_rejects_bad_level(gpu):
+    rng = np.random.default_rng(3)
+    dims = (40, 30, 20)
+    n = 20000
+    coords = np.stack([rng.integers(0, d, n) for d in dims]).astype(np.int32)
+    coords[0, : n // 2] = 7  # one heavy slice
+    coords[1, : n // 3] = 3  # one heavy level-1 coordinate across slices
+    vals = rng.uniform(-1, 1, n).astype(np.float32)
+    B = tt.Csf.from_coo(dims, coords, vals)
+    X = np.zeros(dims)
+    np.add.at(X, tuple(coords.astype(np.int64)), vals.astype(np.float64))
+    R = 32
+    facs = [rng.uniform(-1, 1, (d, R)).astype(np.float32) for d in dims]
+    fd = [tt.DeviceArray.from_numpy(f) for f in facs]
+    H = tt.DeviceCsf(B)
+    for level in (1, 2):
+        A = tt.DeviceArray.empty((dims[level], R))
+        H.mttkrp_level(level, [None if l == level else fd[l] for l in range(3)], A, R)
+        ref = ocpd.mttkrp_mode(X, [f.astype(np.float64) for f in facs], level)
+        assert np.linalg.norm(A.numpy() - ref) / np.linalg.norm(ref) <= REL_L2_TOL
+    with pytest.raises(tt.TentacoError):
+        H.mttkrp_level(0, fd, tt.DeviceArray.empty((dims[0], R)), R)  # the root: use mttkrp
+    with pytest.raises(tt.TentacoError):
+        H.mttkrp_level(1, fd, tt.DeviceArray.empty((dims[1] - 1, R)), R, a_rows=dims[1] - 1)
+    H.close()
