@@ -51,6 +51,66 @@ def profile_step(args):
     from tentaco_amd import metrics
 
     n = args.warmup + args.steps
+    if args.profile_step == "ttv":
+        import numpy as np
+
+        B2 = metrics.workload_zipf(3, 100_000, 20_000_000, seed=1001)
+        c = np.random.default_rng(1011).uniform(-1, 1, 100_000).astype(np.float32)
+        H2 = tt.DeviceCsf(B2)
+        cd = tt.DeviceArray.from_numpy(c)
+        y = tt.DeviceArray.empty((B2.nodes(1),))
+        for _ in range(n):
+            H2.ttv(cd, y)
+        tt.synchronize()
+        return
+    if args.profile_step == "mttkrp3_100m":
+        B = metrics.workload_zipf(3, 500_000, 100_000_000, seed=1005)
+        R = 32
+        facs = metrics.factors(B, R, 1035)
+        H = tt.DeviceCsf(B)
+        fd = [tt.DeviceArray.from_numpy(f) for f in facs]
+        A = tt.DeviceArray.empty((B.dims[0], R))
+        v = tt.variants().index(args.variant)
+        H.reserve(R, v)
+        for _ in range(n):
+            H.mttkrp(fd, A, R, v)
+        tt.synchronize()
+        return
+    if args.profile_step in ("batch", "pertask"):
+        import numpy as np
+
+        tensors = metrics.workload_batch()
+        R = 32
+        rng = np.random.default_rng(1004)
+        tasks, facs_d, outs, rows, keep = [], [], [], [], []
+        for B in tensors:
+            Fd = [tt.DeviceArray.from_numpy(rng.uniform(-1, 1, (d, R)).astype(np.float32)) for d in B.dims]
+            keep.append(Fd)
+            for m, Bm in enumerate(B.mode_csfs()):
+                rest = [k for k in range(B.order) if k != m]
+                tasks.append(tt.DeviceCsf(Bm))
+                facs_d.append([Fd[k] for k in rest])
+                outs.append(tt.DeviceArray.empty((B.dims[m], R)))
+                rows.append(B.dims[m])
+        if args.profile_step == "pertask":
+            # one launch sequence per (tensor, mode), each with its own tuned variant
+            for H, fd, o in zip(tasks, facs_d, outs):
+                H.autotune(fd, o, R, reps=1)
+            for _ in range(n):
+                for H, fd, o in zip(tasks, facs_d, outs):
+                    H.mttkrp(fd, o, R, tt.VARIANT_AUTO)
+            tt.synchronize()
+            return
+        bt = tt.Batch(tasks, R)
+        bt.set_operands(facs_d, outs, rows)
+        cfg = [int(x) for x in args.variant.split(",")] if args.variant else [1, 128, 64]
+        fam, blk, grp = cfg[:3]
+        bt.configure(fam, blk, grp)
+        bt.capture()
+        for _ in range(n):
+            bt.launch()
+        tt.synchronize()
+        return
     if args.profile_step == "copy":
         src = tt.DeviceArray.empty((COPY_BYTES // 4,))
         dst = tt.DeviceArray.empty((COPY_BYTES // 4,))
@@ -469,7 +529,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 FETCH_SIZE/WRITE_SIZE passes")
-    ap.add_argument("--profile-step", choices=["mttkrp3", "copy"], default=None,
+    ap.add_argument("--profile-step", choices=["mttkrp3", "copy", "ttv", "mttkrp3_100m", "batch",
+                                                "pertask"],
+                    default=None,
                     help=argparse.SUPPRESS)
     ap.add_argument("--variant", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
