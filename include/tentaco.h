@@ -346,6 +346,13 @@ int tentaco_csf_set_selector(tentaco_csf* h, tentaco_mlp* m, float min_margin);
  * the fastest (source SELECTOR_TIMED), a fraction of the sweep's cost. */
 int tentaco_csf_set_selector_fallback(tentaco_csf* h, int top_k);
 
+/* tentaco_mttkrp_dev issued `count` times back to back from the library's
+ * own loop (the reference drivers' trial loop, mttkrp_csf_gpu.h:148, without
+ * per-call host-language overhead between launches). */
+int tentaco_mttkrp_dev_repeat(tentaco_csf* h, int R, const float* const* factors_dev,
+                              float* A_dev, int32_t a_rows, int variant, void* stream,
+                              int count);
+
 /* MTTKRP for the mode held at CSF level `level` (1..order-1) of an uploaded
  * CSF, without a CSF rooted at that mode (one CSF for every mode instead of
  * one per mode): A(j,:) = sum over nonzeros whose level-`level` coordinate

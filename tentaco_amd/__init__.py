@@ -106,6 +106,8 @@ def lib() -> ctypes.CDLL:
         "tentaco_csf_reserve": (c_int, [c_void_p, c_int, c_int]),
         "tentaco_mttkrp_dev": (c_int, [c_void_p, c_int, POINTER(c_void_p), c_void_p, c_int32, c_int, c_void_p]),
         "tentaco_ttv_dev": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+        "tentaco_mttkrp_dev_repeat": (c_int, [c_void_p, c_int, POINTER(c_void_p), c_void_p, c_int32, c_int,
+                                              c_void_p, c_int]),
         "tentaco_mttkrp_level_dev": (c_int, [c_void_p, c_int, c_int, POINTER(c_void_p), c_void_p, c_int32,
                                              c_void_p]),
         "tentaco_csf_autotune": (c_int, [c_void_p, c_int, POINTER(c_void_p), c_void_p, c_int32, c_int,
@@ -555,6 +557,13 @@ class DeviceCsf:
         arr = (POINTER(TacoTensor) * len(tf))(*[ctypes.pointer(t) for t, _ in tf])
         self._keep_call = (tf, tA, kA, arr, factors, A)
         _check(lib().tentaco_mttkrp_host_async(self.h, R, arr, ctypes.byref(tA), variant, stream.h))
+
+    def mttkrp_repeat(self, factors: Sequence[DeviceArray], A: DeviceArray, R: int, count: int,
+                      variant: int = VARIANT_AUTO, stream=None, a_rows: Optional[int] = None):
+        """`count` back-to-back MTTKRP calls issued from the library's loop."""
+        arr = (c_void_p * len(factors))(*[f.ptr for f in factors])
+        rows = self.dims[0] if a_rows is None else a_rows
+        _check(lib().tentaco_mttkrp_dev_repeat(self.h, R, arr, A.ptr, rows, variant, stream, count))
 
     def mttkrp_level(self, level: int, factors: Sequence[Optional[DeviceArray]], A: DeviceArray,
                      R: int, stream=None, a_rows: Optional[int] = None):

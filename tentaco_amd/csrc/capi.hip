@@ -159,7 +159,36 @@ void free_handle(tentaco_csf* h) {
   (void)hipFree(h->ws);
   for (int m = 0; m < kMaxOrder; ++m) (void)hipFree(h->stage_f[m]);
   (void)hipFree(h->stage_a);
+  (void)hipFree(h->d_crd1_of_nnz);
+  (void)hipFree(h->d_nnz_of_root);
   delete h;
+}
+
+// The 3-mode expansion (level-1 coordinate per nonzero, first nonzero per
+// root node) that the slice-row kernel's short-fibre path reads; built once,
+// on `st`.
+int ensure_expand3(tentaco_csf* h, hipStream_t st) {
+  if (h->order != 3 || h->d_crd1_of_nnz) return TENTACO_OK;
+  const int64_t nnz = h->nodes[2];
+  if (nnz <= 0) return TENTACO_OK;
+  int32_t *kq = nullptr, *nzr = nullptr;
+  if (hipMalloc(reinterpret_cast<void**>(&kq), size_t(nnz) * sizeof(int32_t)) != hipSuccess)
+    return fail(TENTACO_ENOMEM, "expansion: hipMalloc of %lld ints failed", (long long)nnz);
+  if (hipMalloc(reinterpret_cast<void**>(&nzr), (size_t(h->nodes[0]) + 1) * sizeof(int32_t)) !=
+      hipSuccess) {
+    (void)hipFree(kq);
+    return fail(TENTACO_ENOMEM, "expansion: hipMalloc failed");
+  }
+  int rc = launch_expand3(h->view(), kq, nzr, st);
+  if (rc) {
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(kq);
+    (void)hipFree(nzr);
+    return rc;
+  }
+  h->d_crd1_of_nnz = kq;
+  h->d_nnz_of_root = nzr;
+  return TENTACO_OK;
 }
 
 // Grow a device buffer to at least `bytes` (contents are not kept).
@@ -533,6 +562,11 @@ int tentaco_csf_reserve(tentaco_csf* h, int R, int variant) {
     if (vr.family < 0) return fail(TENTACO_EINVAL, "variant %d out of range", variant);
     need = std::max(need, mttkrp_workspace_bytes(h->view(), R, vr));
   }
+  // the slice-row kernel's 3-mode expansion, so that no later call allocates
+  int rc = ensure_expand3(h, nullptr);
+  if (!rc) rc = (hipDeviceSynchronize() == hipSuccess) ? TENTACO_OK
+                                                       : fail(TENTACO_EHIP, "synchronize failed");
+  if (rc) return rc;
   return reserve_ws(h, need);
 }
 
@@ -616,6 +650,10 @@ int tentaco_mttkrp_dev(tentaco_csf* h, int R, const float* const* factors_dev, f
   }
   const Variant vr = variant_of(v);
   if (vr.family < 0) return fail(TENTACO_EINVAL, "variant %d out of range", variant);
+  if (vr.family == kSliceRow) {
+    rc = ensure_expand3(h, static_cast<hipStream_t>(stream));
+    if (rc) return rc;
+  }
   const CsfDev B = h->view();
   rc = reserve_ws(h, mttkrp_workspace_bytes(B, R, vr));
   if (rc) return rc;
@@ -700,6 +738,17 @@ static int sweep_impl(tentaco_csf* h, int R, const float* const* factors_dev, fl
     }
     std::sort(t.begin(), t.end());
     times_ms[v] = t[t.size() / 2];
+  }
+  return TENTACO_OK;
+}
+
+int tentaco_mttkrp_dev_repeat(tentaco_csf* h, int R, const float* const* factors_dev,
+                              float* A_dev, int32_t a_rows, int variant, void* stream,
+                              int count) {
+  if (count < 0) return fail(TENTACO_EINVAL, "count = %d", count);
+  for (int i = 0; i < count; ++i) {
+    int rc = tentaco_mttkrp_dev(h, R, factors_dev, A_dev, a_rows, variant, stream);
+    if (rc) return rc;
   }
   return TENTACO_OK;
 }

@@ -24,6 +24,9 @@ struct tentaco_csf {
   size_t stage_f_cap[tt::kMaxOrder] = {};
   float* stage_a = nullptr;
   size_t stage_a_cap = 0;
+  // 3-mode expansion for the short-fibre slice path (built on first use)
+  int32_t* d_crd1_of_nnz = nullptr;
+  int32_t* d_nnz_of_root = nullptr;
   tentaco_mlp* selector = nullptr;  // learned selector for AUTO (not owned)
   float min_margin = 0.05f;         // below it AUTO runs the sweep instead
   int fallback_k = 0;               // > 0: time only the k best-predicted variants
@@ -39,6 +42,8 @@ struct tentaco_csf {
       v.crd[m] = d_crd[m];
     }
     v.vals = d_vals;
+    v.crd1_of_nnz = d_crd1_of_nnz;
+    v.nnz_of_root = d_nnz_of_root;
     return v;
   }
 };

@@ -25,6 +25,10 @@ struct CsfDev {
   const int32_t* pos[kMaxOrder] = {};
   const int32_t* crd[kMaxOrder] = {};
   const float* vals = nullptr;
+  // optional, 3-mode only (built once per handle): the level-1 coordinate of
+  // every nonzero and the first nonzero of every root node (n0 + 1 entries)
+  const int32_t* crd1_of_nnz = nullptr;
+  const int32_t* nnz_of_root = nullptr;
 };
 
 // Factor matrices by level: M[m] for m = 1..order-1, row-major rows x R.
@@ -66,6 +70,11 @@ size_t ttv_workspace_bytes(const CsfDev& B);
 int launch_mttkrp(const CsfDev& B, const FactorsDev& F, int R, float* A,
                   int32_t a_rows, bool dense_root, const Variant& vr,
                   void* ws, hipStream_t stream);
+// Fill the 3-mode expansion arrays of CsfDev (crd1_of_nnz: nnz entries,
+// nnz_of_root: n0 + 1 entries) from the CSF's own pos/crd on `stream`.
+int launch_expand3(const CsfDev& B, int32_t* crd1_of_nnz, int32_t* nnz_of_root,
+                   hipStream_t stream);
+
 // Enqueue the MTTKRP of the mode held at CSF level `level` (1..order-1) on
 // the same CSF: F.M[l] is the factor of level l for every l != level
 // (including the root, l = 0); A (a_rows x R) is zero-filled, then

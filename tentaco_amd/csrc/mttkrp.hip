@@ -583,6 +583,67 @@ __device__ __forceinline__ void slice3_accumulate(const CsfDev& B, const Factors
   }
 }
 
+// Order-3 slice with short fibres (at most 1.25 nonzeros per fibre): lanes
+// take the slice's nonzeros 64 at a time, each with its level-1 coordinate
+// from the per-handle expansion, so the row gathers depend on one coalesced
+// index load instead of the pos -> crd chain of the factorised walk;
+// contribution = val * C(k,:) (.) D(l,:) per nonzero. Row group g takes
+// nonzeros base + t*NG + g through lane shuffles, all row gathers of a
+// chunk in flight at once.
+template <int R>
+__device__ __forceinline__ void slice3_short(const CsfDev& B, const FactorsDev& F, int32_t q0,
+                                             int32_t q1, int lane, int gi, int c, int Rr, int wv,
+                                             int nwv, typename Row<R>::V (&acc)[Row<R>::NV]) {
+  using RO = Row<R>;
+  using V = typename RO::V;
+  constexpr int NV = RO::NV;
+  constexpr int NG = 64 / RO::LPR;
+  constexpr int NT = 64 / NG;
+  const int32_t* __restrict__ kq = B.crd1_of_nnz;
+  const int32_t* __restrict__ lq = B.crd[2];
+  const float* __restrict__ vq = B.vals;
+  const float* __restrict__ Cm = F.M[1];
+  const float* __restrict__ Dm = F.M[2];
+  for (int32_t base = q0 + 64 * wv; base < q1; base += 64 * nwv) {
+    const int32_t q = base + lane;
+    const bool okq = q < q1;
+    const int32_t k = okq ? kq[q] : 0;
+    const int32_t l = okq ? lq[q] : 0;
+    const float v = okq ? vq[q] : 0.f;
+    V cr[NT][NV], dr[NT][NV];
+    float vt[NT];
+#pragma unroll
+    for (int u = 0; u < NT; ++u) {
+      const int src = u * NG + gi;
+      vt[u] = __shfl(v, src);
+      RO::load(cr[u], Cm, __shfl(k, src), c, Rr);
+      RO::load(dr[u], Dm, __shfl(l, src), c, Rr);
+    }
+#pragma unroll
+    for (int u = 0; u < NT; ++u) {
+      V w[NV];
+      RO::zero(w);
+      RO::axpy(w, vt[u], dr[u]);
+      RO::hfma(acc, w, cr[u]);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void expand3_kernel(const int32_t* __restrict__ pos1,
+                                                      const int32_t* __restrict__ pos2,
+                                                      const int32_t* __restrict__ crd1,
+                                                      int32_t n0, int32_t nf,
+                                                      int32_t* __restrict__ kq,
+                                                      int32_t* __restrict__ nzr) {
+  const int64_t tid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t nthr = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t f = tid; f < nf; f += nthr) {
+    const int32_t k = crd1[f];
+    for (int32_t q = pos2[f]; q < pos2[f + 1]; ++q) kq[q] = k;
+  }
+  for (int64_t x = tid; x <= n0; x += nthr) nzr[x] = pos2[pos1[x]];
+}
+
 // One workgroup per root node; its blockDim/64 wavefronts split the slice
 // (64-child chunks round-robin), reduce their rows with xor shuffles, then
 // across wavefronts through LDS; one store per row, no atomics.
@@ -666,6 +727,52 @@ __global__ __launch_bounds__(512) void slice_kernel(CsfDev B, FactorsDev F,
           acc[j].z += o.z;
           acc[j].w += o.w;
         }
+      }
+    }
+    RO::store(A, B.crd0[x], c, Rr, acc);
+  }
+}
+
+// Slice-row kernel for 3-mode tensors whose fibres are short (nnz at most
+// 1.25 x fibres): the slice's nonzeros come straight from the per-handle
+// expansion (slice3_short), then the same wave and LDS reduction and one
+// store per row as slice_kernel.
+template <int R>
+__global__ __launch_bounds__(512) void slice_short_kernel(CsfDev B, FactorsDev F,
+                                                          float* __restrict__ A, int Rr) {
+  using RO = Row<R>;
+  using V = typename RO::V;
+  constexpr int NV = RO::NV;
+  constexpr int LPR = RO::LPR;
+  __shared__ __align__(16) float red[8][Lay<R>::kMaxRank];
+  const int32_t x = blockIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int nwv = blockDim.x >> 6;
+  const int gi = lane / LPR;
+  const int c = lane % LPR;
+  V acc[NV];
+  RO::zero(acc);
+  slice3_short<R>(B, F, B.nnz_of_root[x], B.nnz_of_root[x + 1], lane, gi, c, Rr, wv, nwv, acc);
+  RO::wave_sum(acc);
+  if (nwv == 1) {
+    if (gi == 0) RO::store(A, B.crd0[x], c, Rr, acc);
+    return;
+  }
+  if (gi == 0) {
+#pragma unroll
+    for (int j = 0; j < NV; ++j) *reinterpret_cast<float4*>(&red[wv][RO::col(j, c)]) = acc[j];
+  }
+  __syncthreads();
+  if (wv == 0 && gi == 0) {
+    for (int w = 1; w < nwv; ++w) {
+#pragma unroll
+      for (int j = 0; j < NV; ++j) {
+        const float4 o = *reinterpret_cast<const float4*>(&red[w][RO::col(j, c)]);
+        acc[j].x += o.x;
+        acc[j].y += o.y;
+        acc[j].z += o.z;
+        acc[j].w += o.w;
       }
     }
     RO::store(A, B.crd0[x], c, Rr, acc);
@@ -882,6 +989,16 @@ int launch_typed(const CsfDev& B, const FactorsDev& F, int Rr, float* A,
     }
     if (B.nodes[0] > 0) {
       ktimer_begin(st);
+      if constexpr (N == 3 && R != 0) {
+        // short fibres (nnz <= 1.25 x fibres) with the expansion built: the
+        // kernel that skips the pos -> crd chain
+        if (B.crd1_of_nnz && int64_t(B.nnz) * 4 <= int64_t(B.nodes[1]) * 5) {
+          slice_short_kernel<R><<<B.nodes[0], vr.block, 0, st>>>(B, F, A, Rr);
+          TT_HIP_CHECK(hipGetLastError());
+          ktimer_end(st, "slice_kernel");
+          return TENTACO_OK;
+        }
+      }
       slice_kernel<N, R><<<B.nodes[0], vr.block, 0, st>>>(B, F, A, Rr);
       TT_HIP_CHECK(hipGetLastError());
       ktimer_end(st, "slice_kernel");
@@ -983,6 +1100,15 @@ int launch_order(const CsfDev& B, const FactorsDev& F, int R, float* A,
 }
 
 }  // namespace
+
+int launch_expand3(const CsfDev& B, int32_t* kq, int32_t* nzr, hipStream_t st) {
+  if (B.order != 3) return fail(TENTACO_EINVAL, "expansion is for 3-mode tensors");
+  const int64_t work = std::max<int64_t>(B.nodes[1], int64_t(B.nodes[0]) + 1);
+  expand3_kernel<<<grid_for(work, 256, 16384), 256, 0, st>>>(B.pos[1], B.pos[2], B.crd[1],
+                                                             B.nodes[0], B.nodes[1], kq, nzr);
+  TT_HIP_CHECK(hipGetLastError());
+  return TENTACO_OK;
+}
 
 int launch_fill(float* p, int64_t n, float value, hipStream_t st) {
   if (n <= 0) return TENTACO_OK;

@@ -246,3 +246,38 @@ def test_repeated_calls_reuse_handle(gpu):
         H.mttkrp(fd, A, 32, v)
         e = oracle.rel_l2(A.numpy(), oracle.mttkrp(B, facs, "taco", fp64=True))
         assert e <= REL_L2_TOL, (it, v, e)
+
+
+@pytest.mark.parametrize("R", [16, 32, 64, 128, 7])
+@pytest.mark.parametrize("compressed_root", [False, True])
+def test_slice_row_short_fibres(gpu, R, compressed_root):
+    """3-mode tensors whose fibres hold about one nonzero take the slice-row
+    kernel's short-fibre path (per-nonzero level-1 coordinates from the
+    handle's expansion); every slice-row block size against the oracle."""
+    rng = np.random.default_rng(R + 100 * int(compressed_root))
+    dims = (300, 2000, 2000)
+    n = 20000
+    coords = np.stack([rng.integers(0, d, n) for d in dims]).astype(np.int32)
+    if compressed_root:
+        coords[0] = (coords[0] // 3) * 3  # two thirds of the slices are empty
+    vals = rng.uniform(-1, 1, n).astype(np.float32)
+    B = tt.Csf.from_coo(dims, coords, vals)
+    assert B.nnz * 4 <= B.nodes(1) * 5  # the short-fibre regime
+    facs = [rng.uniform(-1, 1, (d, R)).astype(np.float32) for d in dims[1:]]
+    ref = oracle.mttkrp(B, facs, "taco", fp64=True)
+    D = B  # keeps the arrays the compressed view below shares alive
+    if compressed_root:
+        # the same tensor with only its non-empty slices listed at the root
+        nonempty = np.flatnonzero(np.diff(D.pos[1])).astype(np.int32)
+        pos1 = np.append(D.pos[1][nonempty], D.pos[1][-1]).astype(np.int32)
+        B = tt.Csf(3, dims, [np.array([0, nonempty.size], np.int32), pos1, D.pos[2]],
+                   [nonempty, D.crd[1], D.crd[2]], D.vals)
+    H = tt.DeviceCsf(B)
+    fd = [tt.DeviceArray.from_numpy(f) for f in facs]
+    names = tt.variants()
+    for vname in ("sr_b128", "sr_b256", "sr_b512"):
+        A = tt.DeviceArray.from_numpy(np.full((dims[0], R), 3.0, np.float32))
+        H.mttkrp(fd, A, R, names.index(vname))
+        e = oracle.rel_l2(A.numpy(), ref)
+        assert e <= REL_L2_TOL, (vname, R, compressed_root, e)
+    H.close()
