@@ -995,7 +995,7 @@ int launch_typed(const CsfDev& B, const FactorsDev& F, int Rr, float* A,
         if (B.crd1_of_nnz && int64_t(B.nnz) * 4 <= int64_t(B.nodes[1]) * 5) {
           slice_short_kernel<R><<<B.nodes[0], vr.block, 0, st>>>(B, F, A, Rr);
           TT_HIP_CHECK(hipGetLastError());
-          ktimer_end(st, "slice_kernel");
+          ktimer_end(st, "slice_short_kernel");
           return TENTACO_OK;
         }
       }
