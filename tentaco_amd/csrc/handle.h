@@ -27,6 +27,9 @@ struct tentaco_csf {
   // 3-mode expansion for the short-fibre slice path (built on first use)
   int32_t* d_crd1_of_nnz = nullptr;
   int32_t* d_nnz_of_root = nullptr;
+  // fibre-start bitmap and unit table of the TTV unit kernel (built on the
+  // first TTV call outside stream capture)
+  tt::TtvPlan ttv;
   tentaco_mlp* selector = nullptr;  // learned selector for AUTO (not owned)
   float min_margin = 0.05f;         // below it AUTO runs the sweep instead
   int fallback_k = 0;               // > 0: time only the k best-predicted variants

@@ -85,6 +85,38 @@ int launch_mttkrp_level(const CsfDev& B, const FactorsDev& F, int level, int R, 
 int launch_ttv(const CsfDev& B, const float* c, float* y, void* ws,
                hipStream_t stream);
 
+// Per-handle structure of the unit TTV kernel, built once from the CSF:
+// a bitmap with one bit per nonzero that marks the first nonzero of every
+// level N-2 node (fibre), the fibre holding the first nonzero of every
+// 512-nonzero unit, the table where waves leave the pieces of fibres that
+// cross their ranges, and the window of c cached in LDS.
+struct TtvPlan {
+  bool built = false;
+  bool ok = false;             // every fibre is non-empty: the unit kernel applies
+  void* mem = nullptr;         // one allocation holding the arrays below
+  uint32_t* flags = nullptr;   // nunits * 16 words
+  int32_t* unit_fid = nullptr; // nunits entries
+  int32_t* bnd_fid = nullptr;  // 2 per wave (max_waves)
+  float* bnd_val = nullptr;
+  int32_t nunits = 0;
+  int32_t max_waves = 0;
+  int32_t cdim = 0;            // entries of c read (largest leaf coordinate + 1)
+  int32_t k0 = 0;              // c[k0 .. k0 + lc) is staged in LDS
+  int32_t lc = 0;
+  int lds_floats = 0;          // LDS instance (floats of c per workgroup)
+  int grid = 0;                // workgroups per launch
+};
+// Build the plan on `stream` and wait for it (allocates; not for capture).
+// cdim is how many entries of c the kernel may read (largest leaf
+// coordinate + 1); beyond 2^30 - 8 the plan does not apply.
+int build_ttv_plan(const CsfDev& B, int32_t cdim, TtvPlan* plan, hipStream_t stream);
+void free_ttv_plan(TtvPlan* plan);
+// Enqueue one TTV through the unit kernel (plan.ok must hold): every entry of
+// y is written exactly once, with no atomics on y, so results are bitwise
+// reproducible.
+int launch_ttv_units(const CsfDev& B, const TtvPlan& plan, const float* c, float* y,
+                     hipStream_t stream);
+
 // One task of a batched persistent launch (a CSF with its factors, output
 // and bracket workspace).
 struct BatchTaskDev {

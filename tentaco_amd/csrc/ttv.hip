@@ -3,6 +3,13 @@
 // are the innermost fibers. Replaces ttv_kernel / ttv_csf_gpu_taco
 // (tensor_kernels/ttv_csf_gpu_taco.h:2-121).
 //
+// Two kernels. The unit kernel (below, ttv_unit_kernel) is the main path: it
+// reads a per-handle fibre-start bitmap instead of searching pos, keeps a
+// window of c in LDS, writes every y entry exactly once (no zero fill, no
+// atomics) and needs every fibre to be non-empty. The bracket kernel
+// (ttv_kernel) covers the rest (CSFs with empty fibres, e.g. a 2-mode CSF
+// with empty rows, and calls captured before a handle's plan exists):
+//
 // Work split: each workgroup owns 8*BS consecutive nonzeros, each lane 8
 // consecutive ones (two dwordx4 loads of crd and of vals per lane, fully
 // coalesced per wavefront). The nonzero loads and the c gathers are issued
@@ -16,7 +23,9 @@
 // zero-filled output. A prep launch zero-fills y and scatters, per fiber, the
 // workgroup boundaries that fall inside it (no binary-search pre-pass).
 
+#include <algorithm>
 #include <climits>
+#include <vector>
 
 #include "internal.h"
 #include "rowops.h"
@@ -160,6 +169,227 @@ __global__ __launch_bounds__(kTtvBlock) void ttv_kernel(
   else if (is_last) emit(f, incl, true);
 }
 
+// ---- unit kernel ---------------------------------------------------------
+// Each wave owns a contiguous range of 512-nonzero units and walks it in
+// order, 8 nonzeros per lane per unit. The fibre of a lane's first nonzero is
+// the unit's first fibre plus the fibre starts before it in the unit: an
+// exclusive prefix count of the lanes' start bits (per-lane popcounts 0..8,
+// summed over lanes as 4 bit-plane ballots). A lane closes the fibres that
+// start and end inside it with a plain store; fibres spanning lanes are
+// summed by a segmented scan keyed by fibre id, and the fibre still open at
+// the end of a unit is carried in registers into the next. Fibres crossing
+// the wave's range bounds leave their pieces in a per-wave table (head:
+// the fibre that began before the range; tail: the one open at its end),
+// which a second small launch folds into y. The c entries inside the
+// plan's window are read from LDS, the rest from global memory.
+constexpr int kUnit = 512;
+constexpr int kUnitWaves = 16;
+constexpr int kUnitThreads = 64 * kUnitWaves;
+constexpr uint32_t kOutOfRange = 0xFFFFFFF0u;  // buffer offset past any c (c < 2^30 - 8 entries)
+constexpr int32_t kUnitMaxC = (1 << 30) - 8;
+
+__global__ __launch_bounds__(256) void ttv_plan_kernel(
+    const int32_t* __restrict__ pos, int32_t nf, uint32_t* __restrict__ flags,
+    int32_t* __restrict__ unit_fid, uint32_t* __restrict__ bad) {
+  const int64_t tid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t nthr = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t f = tid; f < nf; f += nthr) {
+    const int32_t s = pos[f];
+    const int32_t e = pos[f + 1];
+    if (s >= e) {
+      atomicOr(bad, 1u);
+      continue;
+    }
+    atomicOr(flags + (s >> 5), 1u << (s & 31));
+    for (int64_t u = (int64_t(s) + kUnit - 1) / kUnit; u * kUnit < e; ++u)
+      unit_fid[u] = int32_t(f);
+  }
+}
+
+// histogram of the leaf coordinates in bins of 2^shift (picks the c window)
+__global__ __launch_bounds__(256) void ttv_hist_kernel(
+    const int32_t* __restrict__ crd, int32_t nnz, int shift, uint32_t* __restrict__ hist) {
+  const int64_t tid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t nthr = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t p = tid; p < nnz; p += nthr) atomicAdd(hist + (crd[p] >> shift), 1u);
+}
+
+// exclusive prefix sum over the wave of cnt in 0..15
+__device__ inline int32_t wave_excl_count(uint32_t cnt) {
+  int32_t r = 0;
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    const uint64_t m = __ballot((cnt >> b) & 1u);
+    r += int32_t(__builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32),
+                                           __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u)))
+         << b;
+  }
+  return r;
+}
+
+template <int LC>
+__global__ __launch_bounds__(kUnitThreads) void ttv_unit_kernel(
+    const int32_t* __restrict__ crd, const float* __restrict__ vals,
+    const uint32_t* __restrict__ flags, const int32_t* __restrict__ unit_fid,
+    const float* __restrict__ cv, int32_t cdim, int32_t k0, int32_t lc, float* __restrict__ y,
+    int32_t nnz, int32_t nunits, int32_t* __restrict__ bnd_fid, float* __restrict__ bnd_val) {
+  __shared__ float s_c[LC];
+  // c through a buffer descriptor: lanes served from LDS pass an offset past
+  // the end, which returns 0 without a memory access, so the LDS and global
+  // gathers of a lane need no branch between them
+  const __amdgpu_buffer_rsrc_t crs = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<float*>(cv), int16_t(0), int32_t(uint32_t(cdim) * 4u), 0x00020000);
+  // stage the window: batches of 16 independent loads per thread
+  for (int32_t base = 0; base < lc; base += 16 * kUnitThreads) {
+    float t[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int32_t i = base + j * kUnitThreads + int32_t(threadIdx.x);
+      t[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
+          crs, i < lc ? uint32_t(k0 + i) * 4u : kOutOfRange, 0, 0));
+    }
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int32_t i = base + j * kUnitThreads + int32_t(threadIdx.x);
+      if (i < lc) s_c[i] = t[j];
+    }
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int32_t W = int32_t(gridDim.x) * kUnitWaves;
+  const int32_t w = int32_t(blockIdx.x) * kUnitWaves + int32_t(threadIdx.x >> 6);
+  const int32_t ub = int32_t(int64_t(nunits) * w / W);
+  const int32_t ue = int32_t(int64_t(nunits) * (w + 1) / W);
+  int32_t cfid = -1;  // fibre open at the end of the previous unit
+  float cval = 0.f;   // its sum so far
+  int32_t hfid = -1;  // fibre that began before this range (its pieces go to the table)
+  float hval = 0.f;
+  bool hset = false;  // (per lane) this lane closed hfid
+  if (ub < ue) {
+    const int32_t P = ub * kUnit;
+    const bool starts = (flags[P >> 5] >> (P & 31)) & 1u;
+    if (!starts) {
+      cfid = unit_fid[ub];
+      hfid = cfid;
+    }
+  }
+  auto emit = [&](int32_t fid, float val) {
+    if (fid == hfid) {
+      hval = val;
+      hset = true;
+    } else {
+      y[fid] = val;
+    }
+  };
+  for (int32_t u = ub; u < ue; ++u) {
+    const int32_t P = u * kUnit;
+    const int32_t p0 = P + lane * 8;
+    const bool active = p0 < nnz;
+    int32_t k[8];
+    float v[8];
+    if (p0 + 8 <= nnz) {
+#pragma unroll
+      for (int h = 0; h < 8; h += 4) {
+        const int4 kk = *reinterpret_cast<const int4*>(crd + p0 + h);
+        const float4 vv = *reinterpret_cast<const float4*>(vals + p0 + h);
+        k[h] = kk.x; k[h + 1] = kk.y; k[h + 2] = kk.z; k[h + 3] = kk.w;
+        v[h] = vv.x; v[h + 1] = vv.y; v[h + 2] = vv.z; v[h + 3] = vv.w;
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const bool ok = p0 + j < nnz;
+        k[j] = ok ? crd[p0 + j] : -1;  // gathers 0 (out-of-range offset)
+        v[j] = ok ? vals[p0 + j] : 0.f;
+      }
+    }
+    const uint32_t fb = active ? (flags[p0 >> 5] >> (p0 & 31)) & 0xFFu : 0u;
+    float x[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int32_t kk = k[j] - k0;
+      const bool win = k[j] >= 0 && uint32_t(kk) < uint32_t(lc);
+      const float g = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
+          crs, (win || k[j] < 0) ? kOutOfRange : uint32_t(k[j]) * 4u, 0, 0));
+      const float l = s_c[win ? kk : 0];
+      x[j] = v[j] * (win ? l : g);
+    }
+    const uint32_t s0 = uint32_t(__builtin_amdgcn_readfirstlane(int(fb))) & 1u;
+    const int32_t fid0 = unit_fid[u] + wave_excl_count(uint32_t(__popc(fb))) + int32_t(fb & 1u) -
+                         int32_t(s0);
+    // the carried fibre ends at the unit boundary, or continues in lane 0
+    if (s0 && cfid >= 0 && lane == 0) emit(cfid, cval);
+    float sum = (lane == 0 && !s0 && cfid >= 0) ? cval : 0.f;
+    float head = 0.f;
+    bool closed = false;
+    int32_t f = fid0;
+#pragma unroll
+    for (int j = 1; j < 8; ++j) {
+      sum += x[j - 1];
+      if ((fb >> j) & 1u) {
+        if (closed) y[f] = sum;  // began and ended inside this lane
+        else head = sum;
+        closed = true;
+        ++f;
+        sum = 0.f;
+      }
+    }
+    sum += x[7];
+    // segmented inclusive scan of the lanes' open (last) fibres
+    const int32_t key = active ? f : INT_MAX;
+    float incl = active ? sum : 0.f;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const float ov = __shfl_up(incl, d);
+      const int32_t ok = __shfl_up(key, d);
+      if (lane >= d && ok == key) incl += ov;
+    }
+    const float prev = __shfl_up(incl, 1);
+    const uint32_t nb0 = uint32_t(__shfl_down(int(fb), 1)) & 1u;
+    const int32_t lastl = min(63, (nnz - 1 - P) >> 3);
+    if (active) {
+      if (closed) emit(fid0, head + ((!(fb & 1u) && lane > 0) ? prev : 0.f));
+      if (lane < lastl && nb0) emit(f, incl);
+    }
+    cfid = __shfl(key, lastl);
+    cval = __shfl(incl, lastl);
+  }
+  // this range's entries of the crossing-fibre table
+  const uint64_t hm = __ballot(hset);
+  float hsum = 0.f;
+  if (hm) hsum = __shfl(hval, int(__ffsll(static_cast<unsigned long long>(hm)) - 1));
+  if (lane == 0) {
+    bnd_fid[2 * w] = hm ? hfid : -1;
+    bnd_val[2 * w] = hsum;
+    bnd_fid[2 * w + 1] = cfid;
+    bnd_val[2 * w + 1] = cval;
+  }
+}
+
+// y(f) = sum of the crossing-table entries of f, for every fibre in the
+// table (entries in position order, -1 = none); runs after the unit kernel,
+// so the launch boundary makes its entries visible (no fences between
+// workgroups on different XCDs)
+__global__ __launch_bounds__(256) void ttv_fold_kernel(
+    const int32_t* __restrict__ bnd_fid, const float* __restrict__ bnd_val, int32_t E,
+    float* __restrict__ y) {
+  const int32_t e = int32_t(blockIdx.x) * 256 + int32_t(threadIdx.x);
+  if (e >= E) return;
+  const int32_t f = bnd_fid[e];
+  if (f < 0) return;
+  int32_t q = e - 1;
+  while (q >= 0 && bnd_fid[q] < 0) --q;
+  if (q >= 0 && bnd_fid[q] == f) return;  // not the first entry of f
+  float s = bnd_val[e];
+  for (int32_t r = e + 1; r < E; ++r) {
+    const int32_t g = bnd_fid[r];
+    if (g < 0) continue;
+    if (g != f) break;
+    s += bnd_val[r];
+  }
+  y[f] = s;
+}
+
 }  // namespace
 
 size_t ttv_workspace_bytes(const CsfDev& B) {
@@ -187,6 +417,143 @@ int launch_ttv(const CsfDev& B, const float* c, float* y, void* ws,
   TT_HIP_CHECK(hipGetLastError());
   ktimer_end(st, "ttv_kernel");
   return TENTACO_OK;
+}
+
+}  // namespace tt
+
+namespace tt {
+namespace {
+
+constexpr int kTtvLdsMax = 38912;  // 152 KB of c per workgroup, one workgroup per CU
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+template <int LC>
+int launch_units_typed(const CsfDev& B, const TtvPlan& P, const float* c, float* y,
+                       hipStream_t st) {
+  ktimer_begin(st);
+  ttv_unit_kernel<LC><<<P.grid, kUnitThreads, 0, st>>>(
+      B.crd[B.order - 1], B.vals, P.flags, P.unit_fid, c, P.cdim, P.k0, P.lc, y, B.nnz, P.nunits,
+      P.bnd_fid, P.bnd_val);
+  TT_HIP_CHECK(hipGetLastError());
+  ktimer_end(st, "ttv_unit_kernel");
+  const int32_t E = 2 * P.grid * kUnitWaves;
+  ttv_fold_kernel<<<(E + 255) / 256, 256, 0, st>>>(P.bnd_fid, P.bnd_val, E, y);
+  TT_HIP_CHECK(hipGetLastError());
+  return TENTACO_OK;
+}
+
+}  // namespace
+
+void free_ttv_plan(TtvPlan* P) {
+  if (!P) return;
+  (void)hipFree(P->mem);
+  *P = TtvPlan{};
+}
+
+int build_ttv_plan(const CsfDev& B, int32_t cdim, TtvPlan* P, hipStream_t st) {
+  free_ttv_plan(P);
+  P->built = true;
+  if (B.order < 2 || B.order > kMaxOrder || B.nnz <= 0 || cdim <= 0 || cdim > kUnitMaxC)
+    return TENTACO_OK;
+  P->cdim = cdim;
+  const int32_t nf = B.nodes[B.order - 2];
+  int dev = 0, cus = 0;
+  TT_HIP_CHECK(hipGetDevice(&dev));
+  TT_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const int lds = cdim <= 4096 ? 4096 : cdim <= 16384 ? 16384 : kTtvLdsMax;
+  const int per_cu = size_t(lds) * sizeof(float) <= 80 * 1024 ? 2 : 1;
+  const int32_t nunits = int32_t((int64_t(B.nnz) + kUnit - 1) / kUnit);
+  int64_t grid = nunits / kUnitWaves;
+  grid = std::max<int64_t>(1, std::min<int64_t>(grid, int64_t(std::max(cus, 1)) * per_cu));
+  const int32_t max_waves = int32_t(grid) * kUnitWaves;
+  const size_t words = size_t(nunits) * (kUnit / 32);
+  const size_t o_flags = 0;
+  const size_t o_unit = o_flags + align256(words * sizeof(uint32_t));
+  const size_t o_bfid = o_unit + align256(size_t(nunits) * sizeof(int32_t));
+  const size_t o_bval = o_bfid + align256(size_t(2) * max_waves * sizeof(int32_t));
+  const size_t o_bad = o_bval + align256(size_t(2) * max_waves * sizeof(float));
+  const size_t total = o_bad + 256;
+  void* mem = nullptr;
+  if (hipMalloc(&mem, total) != hipSuccess)
+    return fail(TENTACO_ENOMEM, "TTV plan: hipMalloc of %zu bytes failed", total);
+  char* base = static_cast<char*>(mem);
+  P->mem = mem;
+  P->flags = reinterpret_cast<uint32_t*>(base + o_flags);
+  P->unit_fid = reinterpret_cast<int32_t*>(base + o_unit);
+  P->bnd_fid = reinterpret_cast<int32_t*>(base + o_bfid);
+  P->bnd_val = reinterpret_cast<float*>(base + o_bval);
+  uint32_t* bad = reinterpret_cast<uint32_t*>(base + o_bad);
+  P->nunits = nunits;
+  P->max_waves = max_waves;
+  P->grid = int(grid);
+  P->lds_floats = lds;
+  TT_HIP_CHECK(hipMemsetAsync(P->flags, 0, words * sizeof(uint32_t), st));
+  TT_HIP_CHECK(hipMemsetAsync(bad, 0, sizeof(uint32_t), st));
+  {
+    int64_t g = (int64_t(nf) + 255) / 256;
+    g = std::max<int64_t>(1, std::min<int64_t>(g, 16384));
+    ttv_plan_kernel<<<int(g), 256, 0, st>>>(B.pos[B.order - 1], nf, P->flags, P->unit_fid, bad);
+    TT_HIP_CHECK(hipGetLastError());
+  }
+  P->k0 = 0;
+  P->lc = std::min(cdim, lds);
+  if (cdim > lds) {
+    // the window of `lds` consecutive entries of c that the most nonzeros
+    // gather from (first best on ties), from a histogram of leaf coordinates
+    int shift = 5;
+    while ((int64_t(cdim) >> shift) + 1 > 65536) ++shift;
+    const int64_t nbins = (int64_t(cdim) >> shift) + 1;
+    const int64_t wb = std::max<int64_t>(1, lds >> shift);
+    uint32_t* hist = nullptr;
+    if (hipMalloc(reinterpret_cast<void**>(&hist), size_t(nbins) * sizeof(uint32_t)) != hipSuccess) {
+      (void)hipStreamSynchronize(st);
+      free_ttv_plan(P);
+      P->built = true;
+      return fail(TENTACO_ENOMEM, "TTV plan: histogram allocation failed");
+    }
+    std::vector<uint32_t> h(static_cast<size_t>(nbins));
+    hipError_t e = hipMemsetAsync(hist, 0, size_t(nbins) * sizeof(uint32_t), st);
+    if (e == hipSuccess) {
+      int64_t g = std::max<int64_t>(1, std::min<int64_t>((int64_t(B.nnz) + 255) / 256, 8192));
+      ttv_hist_kernel<<<int(g), 256, 0, st>>>(B.crd[B.order - 1], B.nnz, shift, hist);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(h.data(), hist, size_t(nbins) * sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(hist);
+    TT_HIP_CHECK(e);
+    uint64_t run = 0, best = 0;
+    int64_t best_b = 0;
+    for (int64_t b = 0; b < nbins; ++b) {
+      run += h[size_t(b)];
+      if (b >= wb) run -= h[size_t(b - wb)];
+      const int64_t start = std::max<int64_t>(0, b - wb + 1);
+      if (run > best) {
+        best = run;
+        best_b = start;
+      }
+    }
+    P->k0 = int32_t(std::min<int64_t>(best_b << shift, int64_t(cdim) - 1));
+    P->lc = int32_t(std::min<int64_t>(lds, int64_t(cdim) - P->k0));
+  }
+  uint32_t bad_h = 0;
+  TT_HIP_CHECK(hipMemcpyAsync(&bad_h, bad, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  TT_HIP_CHECK(hipStreamSynchronize(st));
+  P->ok = bad_h == 0;
+  return TENTACO_OK;
+}
+
+int launch_ttv_units(const CsfDev& B, const TtvPlan& P, const float* c, float* y,
+                     hipStream_t st) {
+  if (!P.ok) return fail(TENTACO_EINVAL, "TTV unit kernel: plan does not apply");
+  switch (P.lds_floats) {
+    case 4096: return launch_units_typed<4096>(B, P, c, y, st);
+    case 16384: return launch_units_typed<16384>(B, P, c, y, st);
+    case kTtvLdsMax: return launch_units_typed<kTtvLdsMax>(B, P, c, y, st);
+    default: return fail(TENTACO_EINVAL, "TTV unit kernel: bad LDS instance %d", P.lds_floats);
+  }
 }
 
 }  // namespace tt

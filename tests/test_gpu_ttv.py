@@ -99,3 +99,71 @@ def test_ttv_reference_name(gpu):
     fn = tt.lib().ttv_csf_gpu_taco
     fn(ctypes.byref(tA), ctypes.byref(tB), ctypes.byref(tc))
     assert oracle.rel_l2(y, oracle.ttv(B, c, "taco", True)) <= REL_L2_TOL
+
+
+def test_ttv_unit_kernel_window_and_determinism(gpu):
+    """c longer than the LDS window (100k entries) with leaf coordinates
+    concentrated in 60k..80k, so the window starts away from 0 and gathers
+    come from both LDS and global memory; fibres of 1..5000 nonzeros cross
+    wave ranges. Every entry of y is overwritten, and two calls agree
+    bitwise (no atomics on y)."""
+    rng = np.random.default_rng(75)
+    n = 400_000
+    dims = (300, 400, 100_000)
+    coords = np.stack([rng.integers(0, dims[0], n), rng.integers(0, dims[1], n),
+                       np.where(rng.random(n) < 0.8, rng.integers(60_000, 80_000, n),
+                                rng.integers(0, dims[2], n))]).astype(np.int32)
+    # a few very long fibres
+    long = np.stack([np.full(5000, 7), np.full(5000, 9), rng.choice(dims[2], 5000, replace=False)])
+    coords = np.concatenate([coords, long.astype(np.int32)], 1)
+    vals = rng.uniform(-1, 1, coords.shape[1]).astype(np.float32)
+    B = tt.Csf.from_coo(dims, coords, vals)
+    c = rng.uniform(-1, 1, dims[2]).astype(np.float32)
+    H = tt.DeviceCsf(B)
+    cd = tt.DeviceArray.from_numpy(c)
+    y = tt.DeviceArray.from_numpy(np.full(B.nodes(1), np.nan, np.float32))
+    H.ttv(cd, y)
+    a = y.numpy()
+    assert np.all(np.isfinite(a))
+    assert oracle.rel_l2(a, oracle.ttv(B, c, "taco", True)) <= REL_L2_TOL
+    y2 = tt.DeviceArray.from_numpy(np.full(B.nodes(1), -7.0, np.float32))
+    H.ttv(cd, y2)
+    assert np.array_equal(a, y2.numpy())
+    H.close()
+
+
+@pytest.mark.parametrize("n", [1, 511, 512, 513, 8191, 100_003])
+def test_ttv_unit_kernel_sizes(gpu, n):
+    """nnz around the 512-nonzero unit and the 16-wave workgroup, with small
+    c (whole vector in LDS) and length-1 fibres."""
+    rng = np.random.default_rng(n)
+    dims = (max(1, n // 50), 97, 300)
+    B = random_csf(rng, dims, n)
+    c = rng.uniform(-1, 1, dims[2]).astype(np.float32)
+    got = _ttv_dev(B, c)
+    assert oracle.rel_l2(got, oracle.ttv(B, c, "taco", True)) <= REL_L2_TOL
+    ones = _ttv_dev(tt.Csf(B.order, B.dims, B.pos, B.crd, np.ones_like(B.vals), _owner=B._owner),
+                    np.ones(dims[2], np.float32))
+    assert np.array_equal(ones, np.diff(B.pos[2]).astype(np.float32))
+
+
+def test_ttv_empty_fibres_use_the_bracket_kernel(gpu):
+    """A 2-mode CSF with empty rows (dense root) is outside the unit
+    kernel's plan; the bracket kernel writes zeros there."""
+    rng = np.random.default_rng(76)
+    rows = np.array([0, 2, 2, 5, 9], np.int32)
+    coords = np.stack([rows[rng.integers(0, rows.size, 3000)], rng.integers(0, 700, 3000)])
+    B = tt.Csf.from_coo((12, 700), coords.astype(np.int32), rng.uniform(-1, 1, 3000).astype(np.float32))
+    c = rng.uniform(-1, 1, 700).astype(np.float32)
+    got = _ttv_dev(B, c)
+    ref = oracle.ttv(B, c, "taco", True)
+    assert oracle.rel_l2(got, ref) <= REL_L2_TOL
+    empty = np.setdiff1d(np.arange(12), rows)
+    assert np.all(got[empty] == 0.0)
+    tt.ktimer_start()
+    H = tt.DeviceCsf(B)
+    y = tt.DeviceArray.empty((12,))
+    H.ttv(tt.DeviceArray.from_numpy(c), y)
+    total, k, name = tt.ktimer_stop()
+    assert name == "ttv_kernel"
+    H.close()

@@ -66,5 +66,5 @@ def test_kernel_timer_counts_main_kernels(gpu):
     tt.ktimer_start()
     H.ttv(cd, y)
     total, n, k = tt.ktimer_stop()
-    assert n == 1 and k == "ttv_kernel"
+    assert n == 1 and k == "ttv_unit_kernel"
     H.close()
