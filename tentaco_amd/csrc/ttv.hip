@@ -206,12 +206,14 @@ __global__ __launch_bounds__(256) void ttv_plan_kernel(
   }
 }
 
-// histogram of the leaf coordinates in bins of 2^shift (picks the c window)
+// histogram of every `step`-th leaf coordinate in bins of 2^shift (picks the
+// c window; a sample of about 2^20 nonzeros is enough for that)
 __global__ __launch_bounds__(256) void ttv_hist_kernel(
-    const int32_t* __restrict__ crd, int32_t nnz, int shift, uint32_t* __restrict__ hist) {
+    const int32_t* __restrict__ crd, int32_t nnz, int32_t step, int shift,
+    uint32_t* __restrict__ hist) {
   const int64_t tid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t nthr = int64_t(gridDim.x) * blockDim.x;
-  for (int64_t p = tid; p < nnz; p += nthr) atomicAdd(hist + (crd[p] >> shift), 1u);
+  for (int64_t p = tid * step; p < nnz; p += nthr * step) atomicAdd(hist + (crd[p] >> shift), 1u);
 }
 
 // exclusive prefix sum over the wave of cnt in 0..15
@@ -375,17 +377,38 @@ __global__ __launch_bounds__(256) void ttv_fold_kernel(
     float* __restrict__ y) {
   const int32_t e = int32_t(blockIdx.x) * 256 + int32_t(threadIdx.x);
   if (e >= E) return;
-  const int32_t f = bnd_fid[e];
+  // the entry and its neighbours in one round of loads (runs are short:
+  // a fibre has at most one entry per wave range it touches, and invalid
+  // entries are heads only)
+  auto fid_at = [&](int32_t i) { return (i >= 0 && i < E) ? bnd_fid[i] : -2; };
+  const int32_t fm2 = fid_at(e - 2), fm1 = fid_at(e - 1), f = bnd_fid[e];
+  const int32_t f1 = fid_at(e + 1), f2 = fid_at(e + 2);
+  const float v0 = bnd_val[e];
+  const float v1 = e + 1 < E ? bnd_val[e + 1] : 0.f;
+  const float v2 = e + 2 < E ? bnd_val[e + 2] : 0.f;
   if (f < 0) return;
-  int32_t q = e - 1;
-  while (q >= 0 && bnd_fid[q] < 0) --q;
-  if (q >= 0 && bnd_fid[q] == f) return;  // not the first entry of f
-  float s = bnd_val[e];
-  for (int32_t r = e + 1; r < E; ++r) {
-    const int32_t g = bnd_fid[r];
-    if (g < 0) continue;
-    if (g != f) break;
-    s += bnd_val[r];
+  // the previous valid entry: e-1, else e-2 when e-1 is invalid, else further
+  int32_t prev = fm1 >= 0 ? fm1 : (fm1 == -2 ? -2 : fm2);
+  if (fm1 == -1 && fm2 == -1) {
+    int32_t q = e - 3;
+    while (q >= 0 && bnd_fid[q] < 0) --q;
+    prev = q >= 0 ? bnd_fid[q] : -2;
+  }
+  if (prev == f) return;  // not the first entry of f
+  float s = v0;
+  int32_t r = e + 1;
+  if (f1 == f || f1 == -1) {
+    if (f1 == f) s += v1;
+    r = e + 2;
+    if (f2 == f || f2 == -1) {
+      if (f2 == f) s += v2;
+      for (r = e + 3; r < E; ++r) {
+        const int32_t g = bnd_fid[r];
+        if (g < 0) continue;
+        if (g != f) break;
+        s += bnd_val[r];
+      }
+    }
   }
   y[f] = s;
 }
@@ -515,8 +538,10 @@ int build_ttv_plan(const CsfDev& B, int32_t cdim, TtvPlan* P, hipStream_t st) {
     std::vector<uint32_t> h(static_cast<size_t>(nbins));
     hipError_t e = hipMemsetAsync(hist, 0, size_t(nbins) * sizeof(uint32_t), st);
     if (e == hipSuccess) {
-      int64_t g = std::max<int64_t>(1, std::min<int64_t>((int64_t(B.nnz) + 255) / 256, 8192));
-      ttv_hist_kernel<<<int(g), 256, 0, st>>>(B.crd[B.order - 1], B.nnz, shift, hist);
+      const int32_t step = std::max<int32_t>(1, B.nnz >> 20);
+      const int64_t n = (int64_t(B.nnz) + step - 1) / step;
+      int64_t g = std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 4096));
+      ttv_hist_kernel<<<int(g), 256, 0, st>>>(B.crd[B.order - 1], B.nnz, step, shift, hist);
       e = hipGetLastError();
     }
     if (e == hipSuccess)
