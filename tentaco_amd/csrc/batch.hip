@@ -25,6 +25,10 @@ struct tentaco_batch {
   int32_t* d_brk = nullptr;
   size_t brk_bytes = 0;
   tt::BatchLaunch L;
+  std::vector<int64_t> h_fill, h_bnd;  // host prefix tables (ntasks + 1)
+  // side stream and events of the overlapped per-order preps
+  hipStream_t side = nullptr;
+  hipEvent_t ev[tt::kMaxOrder + 1] = {};
   // graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -97,7 +101,10 @@ int configure(tentaco_batch* b, int family, int block, int group) {
                          hipMemcpyHostToDevice));
   TT_HIP_CHECK(hipMemcpy(b->d_item_off, item.data(), sizeof(int64_t) * (b->ntasks + 1),
                          hipMemcpyHostToDevice));
+  b->h_bnd = bnd;
   BatchLaunch& L = b->L;
+  L.h_bnd_off = b->h_bnd.data();
+  L.h_fill_off = b->h_fill.data();
   L.tasks = b->d_tasks;
   L.ntasks = b->ntasks;
   L.bnd_off = b->d_bnd_off;
@@ -162,6 +169,15 @@ int tentaco_batch_create(tentaco_csf* const* tasks, int ntasks, int R, tentaco_b
     delete b;
     return fail(TENTACO_ENOMEM, "counter allocation failed");
   }
+  // the side stream of the per-order preps (default priority); without it
+  // every launch runs in order on the caller's stream
+  bool side_ok = hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) == hipSuccess;
+  for (int k = 0; k <= kMaxOrder && side_ok; ++k)
+    side_ok = hipEventCreateWithFlags(&b->ev[k], hipEventDisableTiming) == hipSuccess;
+  if (side_ok) {
+    b->L.side = b->side;
+    for (int k = 0; k <= kMaxOrder; ++k) b->L.ev[k] = b->ev[k];
+  }
   *out = b;
   return TENTACO_OK;
 }
@@ -198,6 +214,8 @@ int tentaco_batch_set_operands(tentaco_batch* b, const float* const* factors_dev
                          hipMemcpyHostToDevice));
   b->L.fill_off = b->d_fill_off;
   b->L.total_fill = fill[b->ntasks];
+  b->h_fill = fill;
+  b->L.h_fill_off = b->h_fill.data();
   b->have_operands = true;
   const bool configured = b->L.NB != 0;
   return configure(b, configured ? b->L.family : kEdgeParallel, configured ? b->L.block : 256,
@@ -333,6 +351,9 @@ int tentaco_batch_free(tentaco_batch* b) {
   free_tables(b);
   (void)hipFree(b->d_counter);
   (void)hipFree(b->d_fill_off);
+  for (int k = 0; k <= kMaxOrder; ++k)
+    if (b->ev[k]) (void)hipEventDestroy(b->ev[k]);
+  if (b->side) (void)hipStreamDestroy(b->side);
   delete b;
   return TENTACO_OK;
 }

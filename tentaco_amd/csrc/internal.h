@@ -147,6 +147,16 @@ struct BatchLaunch {
   int32_t grp_task[kMaxOrder + 2] = {};
   int32_t grp_item[kMaxOrder + 2] = {};
   int grp_grid[kMaxOrder + 1] = {};
+  // prep of every order group after the first on `side` (default priority),
+  // overlapping the earlier groups' kernels: ev[0] forks side from the launch
+  // stream, ev[N] marks order N's prep done (the launch stream waits on it
+  // before order N's kernel). side == nullptr runs everything in order.
+  hipStream_t side = nullptr;
+  hipEvent_t ev[kMaxOrder + 1] = {};
+  // host copies of fill_off and bnd_off with ntasks + 1 entries (the group
+  // ranges of the per-order preps)
+  const int64_t* h_fill_off = nullptr;
+  const int64_t* h_bnd_off = nullptr;
 };
 
 int launch_batch(const BatchLaunch& L, hipStream_t stream);

@@ -809,36 +809,40 @@ __device__ __forceinline__ int32_t upper_index(const int64_t* __restrict__ off, 
   return lo;
 }
 
+// Prep of the order group holding sorted tasks [t_lo, t_hi): reset that
+// order's work counter, zero the group's outputs (quads [q_lo, q_hi)) and
+// write its brackets (entries [b_lo, b_hi)).
 __global__ __launch_bounds__(256) void batch_prep_kernel(const BatchTaskDev* __restrict__ tasks,
-                                                         int32_t ntasks,
+                                                         int32_t t_lo, int32_t t_hi,
                                                          const int64_t* __restrict__ fill_off,
-                                                         int64_t total_fill,
+                                                         int64_t q_lo, int64_t q_hi,
                                                          const int64_t* __restrict__ bnd_off,
-                                                         int64_t total_bnd,
+                                                         int64_t b_lo, int64_t b_hi,
                                                          int32_t* __restrict__ counter,
                                                          int32_t NB) {
   const int64_t tid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t nthr = int64_t(gridDim.x) * blockDim.x;
-  if (tid <= kMaxOrder) counter[tid] = 0;  // one work counter per order
+  if (tid == 0) *counter = 0;
+  const int32_t nt = t_hi - t_lo;
   // Zero the outputs as float4 quads (every output holds rows x R floats
   // with R a multiple of 4 and starts 16-byte aligned, which the host
   // checks). Each workgroup takes one contiguous range of quads, finds the
   // task of its first quad with one search and walks forward across task
   // boundaries (empty tasks included).
-  const int64_t total_q = total_fill >> 2;
+  const int64_t total_q = q_hi - q_lo;
   const int64_t per = (total_q + gridDim.x - 1) / gridDim.x;
-  const int64_t q0 = int64_t(blockIdx.x) * per;
-  const int64_t q1 = q0 + per < total_q ? q0 + per : total_q;
+  const int64_t q0 = q_lo + int64_t(blockIdx.x) * per;
+  const int64_t q1 = q0 + per < q_hi ? q0 + per : q_hi;
   if (q0 < q1) {
-    int32_t t = upper_index(fill_off, ntasks, q0 << 2);
+    int32_t t = t_lo + upper_index(fill_off + t_lo, nt, q0 << 2);
     int64_t base = fill_off[t] >> 2;
-    int64_t next = t + 1 < ntasks ? fill_off[t + 1] >> 2 : total_q;
+    int64_t next = t + 1 < t_hi ? fill_off[t + 1] >> 2 : q_hi;
     float4* A4 = reinterpret_cast<float4*>(tasks[t].A);
     for (int64_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
       if (q >= next) {
         do {
           ++t;
-          next = t + 1 < ntasks ? fill_off[t + 1] >> 2 : total_q;
+          next = t + 1 < t_hi ? fill_off[t + 1] >> 2 : q_hi;
         } while (q >= next);
         base = fill_off[t] >> 2;
         A4 = reinterpret_cast<float4*>(tasks[t].A);
@@ -846,8 +850,8 @@ __global__ __launch_bounds__(256) void batch_prep_kernel(const BatchTaskDev* __r
       A4[q - base] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
   }
-  for (int64_t i = tid; i < total_bnd; i += nthr) {
-    const int32_t t = upper_index(bnd_off, ntasks, i);
+  for (int64_t i = b_lo + tid; i < b_hi; i += nthr) {
+    const int32_t t = t_lo + upper_index(bnd_off + t_lo, nt, i);
     const BatchTaskDev& T = tasks[t];
     if (T.B.nnz <= 0) continue;
     const int64_t bi = i - bnd_off[t];
@@ -926,21 +930,51 @@ int launch_batch_order(const BatchLaunch& L, hipStream_t st) {
   return TENTACO_OK;
 }
 
+// The prep of order group N (a group without tasks has none).
+int launch_batch_prep(const BatchLaunch& L, int N, hipStream_t st) {
+  const int32_t t_lo = L.grp_task[N], t_hi = L.grp_task[N + 1];
+  if (t_hi <= t_lo) return TENTACO_OK;
+  // the host prefix tables hold ntasks + 1 entries (the last is the total)
+  const int64_t f0 = L.h_fill_off[t_lo], f1 = L.h_fill_off[t_hi];
+  const int64_t b0 = L.h_bnd_off[t_lo], b1 = L.h_bnd_off[t_hi];
+  const int64_t work = std::max<int64_t>((f1 - f0) >> 2, b1 - b0);
+  batch_prep_kernel<<<grid_for(work, 256, 16384), 256, 0, st>>>(
+      L.tasks, t_lo, t_hi, L.fill_off, f0 >> 2, f1 >> 2, L.bnd_off, b0, b1, L.counter + N, L.NB);
+  TT_HIP_CHECK(hipGetLastError());
+  return TENTACO_OK;
+}
+
 template <int R>
 int launch_batch_typed(const BatchLaunch& L, hipStream_t st) {
-  {
-    const int64_t work = std::max<int64_t>(L.total_fill, L.total_bnd);
-    batch_prep_kernel<<<grid_for(work, 256, 16384), 256, 0, st>>>(
-        L.tasks, L.ntasks, L.fill_off, L.total_fill, L.bnd_off, L.total_bnd, L.counter, L.NB);
-    TT_HIP_CHECK(hipGetLastError());
-  }
-  if (L.total_items == 0) return TENTACO_OK;
-  ktimer_begin(st);
-  int rc = launch_batch_order<2, R>(L, st);
-  if (!rc) rc = launch_batch_order<3, R>(L, st);
-  if (!rc) rc = launch_batch_order<4, R>(L, st);
-  if (!rc) rc = launch_batch_order<5, R>(L, st);
+  int first = -1;
+  for (int N = 2; N <= kMaxOrder && first < 0; ++N)
+    if (L.grp_task[N + 1] > L.grp_task[N]) first = N;
+  if (first < 0) return TENTACO_OK;
+  const bool split = L.side != nullptr;
+  int rc = launch_batch_prep(L, first, st);
   if (rc) return rc;
+  if (split) {
+    TT_HIP_CHECK(hipEventRecord(L.ev[0], st));
+    TT_HIP_CHECK(hipStreamWaitEvent(L.side, L.ev[0], 0));
+  }
+  for (int N = first + 1; N <= kMaxOrder; ++N) {
+    if (L.grp_task[N + 1] <= L.grp_task[N]) continue;
+    rc = launch_batch_prep(L, N, split ? L.side : st);
+    if (rc) return rc;
+    if (split) TT_HIP_CHECK(hipEventRecord(L.ev[N], L.side));
+  }
+  ktimer_begin(st);
+  for (int N = first; N <= kMaxOrder; ++N) {
+    if (L.grp_task[N + 1] <= L.grp_task[N]) continue;
+    if (split && N != first) TT_HIP_CHECK(hipStreamWaitEvent(st, L.ev[N], 0));
+    switch (N) {
+      case 2: rc = launch_batch_order<2, R>(L, st); break;
+      case 3: rc = launch_batch_order<3, R>(L, st); break;
+      case 4: rc = launch_batch_order<4, R>(L, st); break;
+      default: rc = launch_batch_order<5, R>(L, st); break;
+    }
+    if (rc) return rc;
+  }
   ktimer_end(st, "batch_kernel");
   return TENTACO_OK;
 }
