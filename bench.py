@@ -670,6 +670,15 @@ def main():
             "cpu_model": _cpu_model(),
         }
         log(f"CPU {cpu['value']} GB/s ({mn:.2f} ms min) on {cpu['cores']} threads")
+        # the reference's own driver setting, OMP_NUM_THREADS=4 (run_gpu.sh:9)
+        all_threads = cpu["cores"]
+        oracle.set_num_threads(4)
+        mn4, md4 = oracle.time_mttkrp(B, facs, "taco", fp64=True, trials=25,
+                                      flush_doubles=64 * 1024 * 1024)
+        oracle.set_num_threads(all_threads)
+        cpu["at_4_threads"] = {"value": round(balg / (mn4 * 1e-3) / 1e9, 3), "min_ms": round(mn4, 4),
+                               "median_ms": round(md4, 4), "cores": 4}
+        log(f"CPU at 4 threads: {cpu['at_4_threads']['value']} GB/s ({mn4:.2f} ms min)")
 
     extra = {
         "pct_hbm_peak": round(100 * value / metrics.HBM_PEAK_GBPS, 3),

@@ -69,6 +69,8 @@ def load():
         L.oracle_time_ttv.argtypes = [c_int, c_int, P, P, c_int, c_int64, POINTER(c_double),
                                       POINTER(c_double)]
         L.oracle_num_threads.restype = c_int
+        L.oracle_set_num_threads.argtypes = [c_int]
+        L.oracle_set_num_threads.restype = None
         L.oracle_search_before.argtypes = [POINTER(c_int32), c_int32, c_int32, c_int32]
         L.oracle_search_before.restype = c_int32
         _lib = L
@@ -189,6 +191,11 @@ def time_ttv(B, c, fp64=True, trials=25, flush_doubles=0):
 
 def num_threads() -> int:
     return int(load().oracle_num_threads())
+
+
+def set_num_threads(n: int) -> None:
+    """OpenMP threads of the timed CPU nests (later calls use n)."""
+    load().oracle_set_num_threads(int(n))
 
 
 # ---------------------------------------------------------------------------

@@ -685,6 +685,12 @@ int oracle_time_ttv(int order, int fp64, const taco_tensor_t* B,
 
 int oracle_num_threads(void) { return omp_get_max_threads(); }
 
+// Thread count of the timed nests (the reference drivers run under
+// OMP_NUM_THREADS, run_gpu.sh:9).
+void oracle_set_num_threads(int n) {
+  if (n > 0) omp_set_num_threads(n);
+}
+
 // taco_binarySearchBefore exposed for the partition tests (ambiguous-match
 // semantics of gpu_library.h:185-205, kept to show why the product does not
 // use them).
