@@ -281,6 +281,94 @@ def run_selector_heldout(tt):
     return out
 
 
+def run_level_modes(tt, metrics, oracle, B, K):
+    """MTTKRP of every non-root mode on one CSF (tentaco_mttkrp_level_dev),
+    against a CSF rooted at that mode with its autotuned variant, R=32."""
+    import numpy as np
+
+    R = 32
+    rng = np.random.default_rng(1012)
+    F = [rng.uniform(-1, 1, (d, R)).astype(np.float32) for d in B.dims]
+    Fd = [tt.DeviceArray.from_numpy(f) for f in F]
+    H = tt.DeviceCsf(B)
+    out = {}
+    for L in range(1, B.order):
+        A = tt.DeviceArray.empty((B.dims[L], R))
+        fl = [None if lv == L else Fd[lv] for lv in range(B.order)]
+
+        def lstep():
+            H.mttkrp_level(L, fl, A, R)
+
+        lstep()
+        lms = time_steps(lstep, K)[0] / K
+        got = A.numpy()
+        order_l = B.mode_levels(L)
+        t0 = time.perf_counter()
+        Bm = B.reorder(order_l)
+        build_s = time.perf_counter() - t0
+        Hm = tt.DeviceCsf(Bm)
+        fm = [Fd[k] for k in order_l[1:]]
+        Am = tt.DeviceArray.empty((B.dims[L], R))
+        best, _ = Hm.autotune(fm, Am, R, reps=3)
+
+        def mstep():
+            Hm.mttkrp(fm, Am, R, tt.VARIANT_AUTO)
+
+        mstep()
+        mms = time_steps(mstep, K)[0] / K
+        ref = oracle.mttkrp(Bm, [F[k] for k in order_l[1:]], "taco", fp64=True)
+        balg = metrics.alg_bytes_mttkrp(Bm, R)
+        out[f"level{L}"] = {
+            "one_csf_ms": round(lms, 4),
+            "one_csf_gbps": round(balg / (lms * 1e-3) / 1e9, 2),
+            "rooted_csf_ms": round(mms, 4),
+            "rooted_csf_gbps": round(balg / (mms * 1e-3) / 1e9, 2),
+            "rooted_variant": tt.variants()[best],
+            "rooted_csf_build_s": round(build_s, 2),
+            "rooted_csf_device_bytes": int(sum(np.asarray(Bm.pos[m]).nbytes + np.asarray(Bm.crd[m]).nbytes
+                                               for m in range(Bm.order)) + Bm.nnz * 4),
+            "rel_l2_one_csf_vs_oracle_fp64": oracle.rel_l2(got, ref),
+            "rel_l2_rooted_vs_oracle_fp64": oracle.rel_l2(Am.numpy(), ref),
+        }
+        log(f"mode at level {L}: one CSF {lms:.3f} ms, rooted CSF {mms:.3f} ms "
+            f"({tt.variants()[best]}), rel_l2 {out[f'level{L}']['rel_l2_one_csf_vs_oracle_fp64']:.1e}")
+        Hm.close()
+    H.close()
+    return out
+
+
+def run_tns_read(tt):
+    """FROSTT .tns parse + COO -> CSF build of a generated 3-mode file
+    (2M nonzeros, 1-based indices, 8 significant digits per value)."""
+    import numpy as np
+    import pandas as pd
+
+    rng = np.random.default_rng(1013)
+    n = 2_000_000
+    dims = (20_000, 30_000, 40_000)
+    coords = np.stack([rng.integers(1, d + 1, n) for d in dims])
+    vals = rng.uniform(-1, 1, n)
+    d = tempfile.mkdtemp(prefix="tt_tns_")
+    path = os.path.join(d, "bench.tns")
+    try:
+        pd.DataFrame({"i": coords[0], "j": coords[1], "k": coords[2], "v": vals}).to_csv(
+            path, sep=" ", header=False, index=False, float_format="%.8g")
+        size = os.path.getsize(path)
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            B, used = tt.Csf.from_tns(path)
+            ts.append(time.perf_counter() - t0)
+        t = min(ts)
+        res = {"nnz_in_file": n, "nnz_after_duplicates": B.nnz, "file_bytes": size,
+               "seconds": round(t, 3), "mb_per_s": round(size / t / 1e6, 1),
+               "mnnz_per_s": round(n / t / 1e6, 2), "mode_order": list(used)}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    log(f"tns: {res['mb_per_s']} MB/s, {res['mnnz_per_s']} M nonzeros/s")
+    return res
+
+
 def run_host_boundary(tt, B, H, facs, R, variant, dev_result, K):
     """The same MTTKRP with operands in host memory (PCIe included, so never
     the headline value): (1) the drop-in host entry point, which uploads the
@@ -718,6 +806,8 @@ def main():
         }
         log(f"TTV 20M: {extra['c2_ttv']['gbps']} GB/s, rel_l2 {e2:.2e}")
         H2.close()
+        extra["level_modes_20m"] = run_level_modes(tt, metrics, oracle, B2, K)
+        extra["tns_read"] = run_tns_read(tt)
 
     if not args.quick:
         extra["c4_batch"] = run_batch_workload(tt, metrics, oracle, K)

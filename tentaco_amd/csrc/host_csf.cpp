@@ -187,6 +187,44 @@ void parallel_for(int64_t n, F&& f) {
   for (auto& x : th) x.join();
 }
 
+// [0, n) split into T contiguous chunks, f(t, begin, end) on T threads.
+template <class F>
+void run_chunks(int T, int64_t n, F&& f) {
+  if (T <= 1) {
+    f(0, int64_t(0), n);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int t = 0; t < T; ++t) th.emplace_back([&, t] { f(t, n * t / T, n * (t + 1) / T); });
+  for (auto& x : th) x.join();
+}
+
+// One stable counting-sort pass on an `nbits`-bit digit: per-chunk
+// histograms, then every chunk scatters to its own offsets. Chunks keep the
+// input order and each scatters in order, so equal digits keep their
+// relative order. move(q, d) places element q at position d.
+template <class DigitF, class MoveF>
+void radix_pass(int64_t n, int nbits, int T, DigitF digit, MoveF move) {
+  const int B = 1 << nbits;
+  std::vector<int64_t> cnt(size_t(T) * size_t(B), 0);
+  run_chunks(T, n, [&](int t, int64_t a, int64_t e) {
+    int64_t* c = cnt.data() + size_t(t) * size_t(B);
+    for (int64_t q = a; q < e; ++q) ++c[digit(q)];
+  });
+  int64_t run = 0;
+  for (int d = 0; d < B; ++d)
+    for (int t = 0; t < T; ++t) {
+      int64_t& c = cnt[size_t(t) * size_t(B) + size_t(d)];
+      const int64_t k = c;
+      c = run;
+      run += k;
+    }
+  run_chunks(T, n, [&](int t, int64_t a, int64_t e) {
+    int64_t* c = cnt.data() + size_t(t) * size_t(B);
+    for (int64_t q = a; q < e; ++q) move(q, c[digit(q)]++);
+  });
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -225,6 +263,12 @@ int build_from_coo(int order, const int32_t* dims, int64_t n, const int32_t* coo
   }
   std::vector<int64_t> perm(static_cast<size_t>(n));
   std::iota(perm.begin(), perm.end(), 0);
+  // parallel LSD radix sort with 11-bit digits (2048 buckets per chunk stay
+  // cache-resident), stable, so duplicates keep their input order
+  constexpr int kDigit = 11;
+  constexpr uint64_t kMask = (1u << kDigit) - 1;
+  const int T = int(std::max<int64_t>(1, std::min<int64_t>(hw_threads(), n / 65536)));
+  std::vector<int64_t> tmp(static_cast<size_t>(n));
   if (total_bits <= 64) {
     std::vector<uint64_t> key(static_cast<size_t>(n));
     parallel_for((n + 65535) / 65536, [&](int64_t chunk) {
@@ -235,38 +279,43 @@ int build_from_coo(int order, const int32_t* dims, int64_t n, const int32_t* coo
         key[size_t(q)] = k;
       }
     });
-    // LSD radix sort, 16 bits per pass, stable
-    std::vector<int64_t> tmp(static_cast<size_t>(n));
     std::vector<uint64_t> ktmp(static_cast<size_t>(n));
-    for (int shift = 0; shift < total_bits; shift += 16) {
-      std::vector<int64_t> cnt(65537, 0);
-      for (int64_t q = 0; q < n; ++q) ++cnt[((key[size_t(q)] >> shift) & 0xFFFF) + 1];
-      for (int i = 0; i < 65536; ++i) cnt[i + 1] += cnt[i];
-      for (int64_t q = 0; q < n; ++q) {
-        const size_t d = size_t(cnt[(key[size_t(q)] >> shift) & 0xFFFF]++);
-        tmp[d] = perm[size_t(q)];
-        ktmp[d] = key[size_t(q)];
-      }
+    for (int shift = 0; shift < total_bits; shift += kDigit) {
+      radix_pass(
+          n, kDigit, T, [&](int64_t q) { return int((key[size_t(q)] >> shift) & kMask); },
+          [&](int64_t q, int64_t d) {
+            tmp[size_t(d)] = perm[size_t(q)];
+            ktmp[size_t(d)] = key[size_t(q)];
+          });
       perm.swap(tmp);
       key.swap(ktmp);
     }
   } else {
-    // LSD radix sort over the modes (last mode first), 16-bit digits, stable
-    std::vector<int64_t> tmp(static_cast<size_t>(n));
+    // over the modes (last mode first), digits of each coordinate
     for (int m = order - 1; m >= 0; --m) {
       const int32_t* cm = coords + int64_t(m) * n;
-      for (int shift = 0; shift < bits[m]; shift += 16) {
-        std::vector<int64_t> cnt(65537, 0);
-        for (int64_t q = 0; q < n; ++q) ++cnt[((uint32_t(cm[perm[size_t(q)]]) >> shift) & 0xFFFF) + 1];
-        for (int i = 0; i < 65536; ++i) cnt[i + 1] += cnt[i];
-        for (int64_t q = 0; q < n; ++q) {
-          const int64_t p = perm[size_t(q)];
-          tmp[size_t(cnt[(uint32_t(cm[p]) >> shift) & 0xFFFF]++)] = p;
-        }
+      for (int shift = 0; shift < bits[m]; shift += kDigit) {
+        radix_pass(
+            n, kDigit, T,
+            [&](int64_t q) { return int((uint32_t(cm[perm[size_t(q)]]) >> shift) & kMask); },
+            [&](int64_t q, int64_t d) { tmp[size_t(d)] = perm[size_t(q)]; });
         perm.swap(tmp);
       }
     }
   }
+  // coordinates and values in sorted order (one parallel gather), so the
+  // level construction below reads sequentially
+  std::vector<int32_t> sc(size_t(order) * size_t(n));
+  std::vector<float> sv(static_cast<size_t>(n));
+  run_chunks(T, n, [&](int, int64_t a, int64_t e) {
+    for (int64_t q = a; q < e; ++q) {
+      const int64_t p = perm[size_t(q)];
+      for (int m = 0; m < order; ++m) sc[size_t(m) * size_t(n) + size_t(q)] = coords[m * n + p];
+      sv[size_t(q)] = vals[p];
+    }
+  });
+  std::vector<int64_t>().swap(perm);
+  std::vector<int64_t>().swap(tmp);
   HostCsf& h = *out;
   h.order = order;
   h.dims.assign(dims, dims + order);
@@ -280,34 +329,30 @@ int build_from_coo(int order, const int32_t* dims, int64_t n, const int32_t* coo
   for (int m = 1; m < order; ++m) h.crd[m].reserve(size_t(n));
   // level-1 nodes per root coordinate (dense root), turned into pos[1] below
   std::vector<int32_t> per_root(size_t(dims[0]) + 1, 0);
-  const int32_t* prev = nullptr;
-  int64_t prev_p = -1;
+  auto C = [&](int m, int64_t q) { return sc[size_t(m) * size_t(n) + size_t(q)]; };
   for (int64_t q = 0; q < n; ++q) {
-    const int64_t p = perm[size_t(q)];
     // smallest level whose coordinate differs from the previous nonzero
     int d = 0;
-    if (prev_p >= 0) {
+    if (q > 0) {
       d = order;
       for (int m = 0; m < order; ++m)
-        if (coords[m * n + p] != coords[m * n + prev_p]) {
+        if (C(m, q) != C(m, q - 1)) {
           d = m;
           break;
         }
     }
     if (d == order) {
-      h.vals.back() += vals[p];  // duplicate coordinate: summed in input order
+      h.vals.back() += sv[size_t(q)];  // duplicate coordinate: summed in input order
       continue;
     }
     for (int m = std::max(d, 1); m < order; ++m) {
-      if (m == 1) ++per_root[size_t(coords[p]) + 1];
+      if (m == 1) ++per_root[size_t(C(0, q)) + 1];
       // a new level-m node; its children start at the current size of level m+1
-      h.crd[m].push_back(coords[m * n + p]);
+      h.crd[m].push_back(C(m, q));
       if (m + 1 < order) h.pos[m + 1].push_back(int32_t(h.crd[m + 1].size()));
     }
-    h.vals.push_back(vals[p]);
-    prev_p = p;
+    h.vals.push_back(sv[size_t(q)]);
   }
-  (void)prev;
   h.pos[1].resize(size_t(dims[0]) + 1);
   int32_t run = 0;
   for (int32_t i = 0; i <= dims[0]; ++i) {

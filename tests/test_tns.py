@@ -115,3 +115,18 @@ def test_tns_rejects_unsupported_and_missing(tmp_path):
         tt.Csf.from_tns(str(g), dims=(4, 4))  # index beyond the given dimension
     with pytest.raises(tt.TentacoError):
         tt.Csf.from_tns(str(g), mode_order=[0, 0])
+
+
+@pytest.mark.parametrize("dims,n", [
+    ((1_000_000, 900_000, 800_000, 700_000, 600_000), 300_000),  # 100 key bits: per-mode digits
+    ((20_000, 30_000, 40_000), 400_000),                          # 45 bits: packed 64-bit keys
+])
+def test_coo_builder_radix_paths_match_oracle(dims, n):
+    """Both sort paths of the library's COO -> CSF builder (packed keys up to
+    64 bits, per-mode digits beyond), each over several thread chunks, are
+    bit-exact against the oracle's builder, duplicates included."""
+    rng = np.random.default_rng(len(dims))
+    coords = np.stack([rng.integers(0, d, n) for d in dims]).astype(np.int32)
+    coords[:, n // 3:n // 3 + 300] = coords[:, :300]
+    vals = rng.uniform(-1, 1, n).astype(np.float32)
+    _same(tt.Csf.from_coo(dims, coords, vals), oracle.csf_from_coo(dims, coords, vals))
