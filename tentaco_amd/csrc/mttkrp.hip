@@ -448,15 +448,19 @@ __device__ __forceinline__ void subtree(const CsfDev& B, const FactorsDev& F,
 // at that mode:
 //   A_L(j,:) = sum over nonzeros with level-L coordinate j of
 //              val * prod_{l != L} M_l(coordinate at level l, :),
-// with M_0 the root mode's factor. One row group per level-(L-1) node p: the
-// product of the factor rows of p and its ancestors (found by upper-bound
-// searches up the pos arrays) times, for every child y of p, the factorised
-// sum of y's subtree below level L, is added atomically into row crd_L[y]
-// of the zero-filled A (children of different parents can share a
-// coordinate).
+// with M_0 the root mode's factor. Every level-L node y contributes
+//   (prod_{l < L} M_l(row of y's level-l ancestor)) * (factorised sum of y's
+//   subtree below L; its value for a nonzero),
+// added atomically into row crd_L[y] of the zero-filled A (nodes of
+// different parents can share a coordinate). Row group g takes chunk g of
+// CH consecutive level-L nodes (grid-stride), so work follows the level-L
+// subtrees rather than the level-(L-1) parents, whose sizes can be Zipf
+// skewed. The ancestors of a chunk's first node are found by upper-bound
+// searches once; after that they advance with the node (nodes are in
+// order), and the ancestor product is recomputed only when one changes.
 template <int N, int L, int R>
 __global__ __launch_bounds__(256) void level_kernel(CsfDev B, FactorsDev F, float* __restrict__ A,
-                                                    int Rr) {
+                                                    int Rr, int32_t CH) {
   using RO = Row<R>;
   using V = typename RO::V;
   constexpr int NV = RO::NV;
@@ -464,23 +468,46 @@ __global__ __launch_bounds__(256) void level_kernel(CsfDev B, FactorsDev F, floa
   const int c = int(threadIdx.x) % LPR;
   const int64_t g0 = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / LPR;
   const int64_t ng = int64_t(gridDim.x) * blockDim.x / LPR;
-  const int32_t np = B.nodes[L - 1];
-  for (int64_t p = g0; p < np; p += ng) {
-    V above[NV];
+  const int32_t ny = B.nodes[L];
+  for (int64_t chunk = g0; chunk * CH < ny; chunk += ng) {
+    const int32_t y0 = int32_t(chunk * CH);
+    const int32_t y1 = int32_t(min(int64_t(ny), chunk * CH + CH));
+    // anc[l]: the level-l ancestor of the current node, l < L
+    int32_t anc[L];
     {
-      const int32_t crd = (L - 1 == 0) ? B.crd0[p] : B.crd[L - 1][p];
-      RO::load(above, F.M[L - 1], crd, c, Rr);
-    }
-    int32_t a = int32_t(p);
+      int32_t x = y0;
 #pragma unroll
-    for (int l = L - 2; l >= 0; --l) {
-      a = search_le(B.pos[l + 1], 0, B.nodes[l] - 1, a);
-      V r[NV];
-      RO::load(r, F.M[l], l == 0 ? B.crd0[a] : B.crd[l][a], c, Rr);
-      RO::hmul(above, r);
+      for (int l = L - 1; l >= 0; --l) {
+        x = search_le(B.pos[l + 1], 0, B.nodes[l] - 1, x);
+        anc[l] = x;
+      }
     }
-    const int32_t y0 = B.pos[L][p], y1 = B.pos[L][p + 1];
+    V above[NV];
+    auto ancestors_product = [&]() {
+      RO::load(above, F.M[0], B.crd0[anc[0]], c, Rr);
+#pragma unroll
+      for (int l = 1; l < L; ++l) {
+        V r[NV];
+        RO::load(r, F.M[l], B.crd[l][anc[l]], c, Rr);
+        RO::hmul(above, r);
+      }
+    };
+    ancestors_product();
     for (int32_t y = y0; y < y1; ++y) {
+      // advance the ancestors so that pos[l+1][anc[l]] <= child < pos[l+1][anc[l]+1]
+      bool moved = false;
+      int32_t x = y;
+#pragma unroll
+      for (int l = L - 1; l >= 0; --l) {
+        if (x >= B.pos[l + 1][anc[l] + 1]) {
+          do {
+            ++anc[l];
+          } while (x >= B.pos[l + 1][anc[l] + 1]);
+          moved = true;
+        }
+        x = anc[l];
+      }
+      if (moved) ancestors_product();
       V out[NV];
       if constexpr (L == N - 1) {
         RO::zero(out);
@@ -1078,13 +1105,15 @@ int launch_level_typed(const CsfDev& B, const FactorsDev& F, int Rr, float* A, i
   const int64_t a_elems = int64_t(a_rows) * Rr;
   fill_kernel<<<grid_for(a_elems / 4 + 1, 256, 16384), 256, 0, st>>>(A, a_elems, 0.f);
   TT_HIP_CHECK(hipGetLastError());
-  const int32_t np = B.nodes[L - 1];
-  if (np <= 0) return TENTACO_OK;
-  const int64_t groups = np;
+  const int32_t ny = B.nodes[L];
+  if (ny <= 0) return TENTACO_OK;
+  // chunks of CH level-L nodes, about 2^16 row groups in all
+  const int32_t CH = int32_t(std::max<int64_t>(4, (int64_t(ny) + 65535) / 65536));
+  const int64_t groups = (int64_t(ny) + CH - 1) / CH;
   const int per_block = 256 / Row<R>::LPR;
   const int grid = grid_for(groups, per_block, 65536);
   ktimer_begin(st);
-  level_kernel<N, L, R><<<grid, 256, 0, st>>>(B, F, A, Rr);
+  level_kernel<N, L, R><<<grid, 256, 0, st>>>(B, F, A, Rr, CH);
   TT_HIP_CHECK(hipGetLastError());
   ktimer_end(st, "level_kernel");
   return TENTACO_OK;
