@@ -9,6 +9,11 @@
 //      inverse by Cholesky (FP64, in LDS), A_t,m = M_t V^-1,
 //   3. column normalisation (2-norm on the first sweep, max(1, max |a|)
 //      afterwards) into lambda_t, and the new Gram A^T A.
+// Steps 2-3 read and write the factor rows twice: one fused pass computes
+// A = M V^-1 and per-block partials of A^T A, of the column max and of the
+// fit's inner product; a per-job kernel reduces them (the 2-norms are the
+// Gram diagonal, and the normalised Gram is (A^T A) ./ (lambda lambda^T));
+// a last pass divides A by lambda.
 // Row-parallel kernels cover every tensor of a step in one launch through a
 // job table; their reductions write per-block partials that a second kernel
 // sums in block order, so results do not depend on scheduling. The fit
@@ -32,6 +37,7 @@ namespace {
 constexpr int kMaxR = 64;
 constexpr int kApplyRows = 64;    // rows per block of the apply / scale kernels
 constexpr int kStatRows = 2048;   // rows per block of the reduction kernels
+constexpr int kUpdRows = 4096;    // rows per block of the fused update kernel
 
 struct CpdJob {
   const float* M = nullptr;  // MTTKRP result, rows x R
@@ -41,9 +47,11 @@ struct CpdJob {
   float* Vinv = nullptr;     // R x R scratch
   const float* Gk[kMaxOrder - 1] = {};
   int nG = 0;
+  double* inner = nullptr;   // <M, A diag(lambda)> of this update, when the fit needs it
   int32_t rows = 0;
   int32_t ablk0 = 0, nablk = 0;  // apply blocks (kApplyRows rows each)
   int32_t sblk0 = 0, nsblk = 0;  // reduction blocks (kStatRows rows each)
+  int32_t ublk0 = 0, nublk = 0;  // update blocks (kUpdRows rows each)
 };
 
 __device__ __forceinline__ int find_job(const int32_t* __restrict__ off, int n, int32_t blk) {
@@ -112,88 +120,6 @@ __global__ __launch_bounds__(256) void hadamard_inverse_kernel(const CpdJob* __r
   }
 }
 
-// A = M V^-1 for kApplyRows rows of one job per block.
-__global__ __launch_bounds__(256) void apply_kernel(const CpdJob* __restrict__ jobs,
-                                                    const int32_t* __restrict__ ablk0, int njobs,
-                                                    int R) {
-  __shared__ float Vs[kMaxR * kMaxR];
-  __shared__ float Ms[kApplyRows][kMaxR + 1];
-  const int j = find_job(ablk0, njobs, blockIdx.x);
-  const CpdJob& J = jobs[j];
-  const int32_t r0 = (int32_t(blockIdx.x) - J.ablk0) * kApplyRows;
-  const int nr = min(kApplyRows, J.rows - r0);
-  for (int e = threadIdx.x; e < R * R; e += blockDim.x) Vs[e] = J.Vinv[e];
-  for (int e = threadIdx.x; e < nr * R; e += blockDim.x)
-    Ms[e / R][e % R] = J.M[int64_t(r0 + e / R) * R + e % R];
-  __syncthreads();
-  for (int e = threadIdx.x; e < nr * R; e += blockDim.x) {
-    const int r = e / R, c = e % R;
-    float s = 0.f;
-    for (int k = 0; k < R; ++k) s = fmaf(Ms[r][k], Vs[k * R + c], s);
-    J.A[int64_t(r0 + r) * R + c] = s;
-  }
-}
-
-// Per reduction block: column statistics of A (sum of squares, or max |a|)
-// into part[block * R + c], in FP64.
-__global__ __launch_bounds__(256) void colstat_kernel(const CpdJob* __restrict__ jobs,
-                                                      const int32_t* __restrict__ sblk0, int njobs,
-                                                      int R, int use_max,
-                                                      double* __restrict__ part) {
-  __shared__ double red[256];
-  const int j = find_job(sblk0, njobs, blockIdx.x);
-  const CpdJob& J = jobs[j];
-  const int32_t r0 = (int32_t(blockIdx.x) - J.sblk0) * kStatRows;
-  const int32_t r1 = min(r0 + kStatRows, J.rows);
-  const int lanes = blockDim.x / R;  // row lanes per column
-  const int c = threadIdx.x % R, l = threadIdx.x / R;
-  double acc = 0.0;
-  if (l < lanes)
-    for (int32_t r = r0 + l; r < r1; r += lanes) {
-      const double a = J.A[int64_t(r) * R + c];
-      acc = use_max ? fmax(acc, fabs(a)) : acc + a * a;
-    }
-  red[threadIdx.x] = acc;
-  __syncthreads();
-  if (threadIdx.x < R) {
-    double s = red[threadIdx.x];
-    for (int q = 1; q < lanes; ++q) {
-      const double v = red[q * R + threadIdx.x];
-      s = use_max ? fmax(s, v) : s + v;
-    }
-    part[int64_t(blockIdx.x) * R + threadIdx.x] = s;
-  }
-}
-
-// lambda from the job's partials (in block order); then A /= lambda.
-__global__ __launch_bounds__(64) void lambda_kernel(const CpdJob* __restrict__ jobs, int R,
-                                                    int use_max,
-                                                    const double* __restrict__ part) {
-  const CpdJob& J = jobs[blockIdx.x];
-  for (int c = threadIdx.x; c < R; c += blockDim.x) {
-    double s = 0.0;
-    for (int b = 0; b < J.nsblk; ++b) {
-      const double v = part[int64_t(J.sblk0 + b) * R + c];
-      s = use_max ? fmax(s, v) : s + v;
-    }
-    const double lam = use_max ? fmax(s, 1.0) : sqrt(s);
-    J.lam[c] = float(lam > 0.0 ? lam : 1.0);
-  }
-}
-
-__global__ __launch_bounds__(256) void scale_kernel(const CpdJob* __restrict__ jobs,
-                                                    const int32_t* __restrict__ ablk0, int njobs,
-                                                    int R) {
-  const int j = find_job(ablk0, njobs, blockIdx.x);
-  const CpdJob& J = jobs[j];
-  const int32_t r0 = (int32_t(blockIdx.x) - J.ablk0) * kApplyRows;
-  const int nr = min(kApplyRows, J.rows - r0);
-  for (int e = threadIdx.x; e < nr * R; e += blockDim.x) {
-    const int64_t q = int64_t(r0) * R + e;
-    J.A[q] = J.A[q] / J.lam[e % R];
-  }
-}
-
 // Gram partials: part[block][i*R + j] = sum over the block's rows of
 // A[r,i] A[r,j] (upper triangle computed, mirrored), FP64 accumulation.
 __global__ __launch_bounds__(256) void gram_partial_kernel(const CpdJob* __restrict__ jobs,
@@ -250,24 +176,285 @@ __global__ __launch_bounds__(256) void gram_reduce_kernel(const CpdJob* __restri
   }
 }
 
-// <M, A diag(lambda)> partials per reduction block (fit).
-__global__ __launch_bounds__(256) void inner_kernel(const CpdJob* __restrict__ jobs,
-                                                    const int32_t* __restrict__ sblk0, int njobs,
-                                                    int R, double* __restrict__ part) {
+// Pass 1 of a mode step, kUpdRows rows of one job per block, in tiles of
+// TR = 1024 / R rows (one float4 of M per thread per tile):
+//   A = M V^-1, written unnormalised;
+//   per-block FP64 partials of A^T A (FP32 within a tile), of the column
+//   max |a|, and of <M, A> (the fit's inner product, since the stored factor
+//   is A / lambda and the model carries lambda).
+// Partials go to pg[block][R*R], pm[block][R], pi[block]; finish_kernel sums
+// them in block order, so the results do not depend on scheduling.
+template <int R>
+__global__ __launch_bounds__(256) void update_kernel(const CpdJob* __restrict__ jobs,
+                                                     const int32_t* __restrict__ ublk0, int njobs,
+                                                     double* __restrict__ pg,
+                                                     double* __restrict__ pm,
+                                                     double* __restrict__ pi) {
+  constexpr int Q = R / 4;           // float4 quads per row
+  constexpr int TR = 256 / Q;        // rows per tile
+  constexpr int EPT = R * R / 256 > 0 ? R * R / 256 : 1;  // Gram entries per thread
+  __shared__ __align__(16) float Vs[R][R];
+  __shared__ float Ms[TR][R + 1];
+  __shared__ float As[TR][R + 1];
   __shared__ double red[256];
-  const int j = find_job(sblk0, njobs, blockIdx.x);
+  const int j = find_job(ublk0, njobs, blockIdx.x);
   const CpdJob& J = jobs[j];
-  const int32_t r0 = (int32_t(blockIdx.x) - J.sblk0) * kStatRows;
-  const int32_t r1 = min(r0 + kStatRows, J.rows);
-  double acc = 0.0;
-  for (int64_t q = int64_t(r0) * R + threadIdx.x; q < int64_t(r1) * R; q += blockDim.x)
-    acc += double(J.M[q]) * double(J.A[q]) * double(J.lam[q % R]);
-  red[threadIdx.x] = acc;
+  const int32_t r0 = (int32_t(blockIdx.x) - J.ublk0) * kUpdRows;
+  const int32_t r1 = min(r0 + kUpdRows, J.rows);
+  for (int e = threadIdx.x; e < R * R; e += 256) Vs[e / R][e % R] = J.Vinv[e];
+  const int rr = int(threadIdx.x) / Q, q = int(threadIdx.x) % Q;
+  double g[EPT];
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) g[k] = 0.0;
+  float cmax[4] = {0.f, 0.f, 0.f, 0.f};
+  double inner = 0.0;
+  for (int32_t t0 = r0; t0 < r1; t0 += TR) {
+    const int32_t row = t0 + rr;
+    const bool ok = row < r1;
+    __syncthreads();  // the previous tile's LDS reads are done (and Vs is loaded)
+    {
+      float4 m4 = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (ok) m4 = *reinterpret_cast<const float4*>(J.M + int64_t(row) * R + 4 * q);
+      Ms[rr][4 * q] = m4.x;
+      Ms[rr][4 * q + 1] = m4.y;
+      Ms[rr][4 * q + 2] = m4.z;
+      Ms[rr][4 * q + 3] = m4.w;
+    }
+    __syncthreads();
+    float o[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 8
+    for (int k = 0; k < R; ++k) {
+      const float mk = Ms[rr][k];
+      const float4 v = *reinterpret_cast<const float4*>(&Vs[k][4 * q]);
+      o[0] = fmaf(mk, v.x, o[0]);
+      o[1] = fmaf(mk, v.y, o[1]);
+      o[2] = fmaf(mk, v.z, o[2]);
+      o[3] = fmaf(mk, v.w, o[3]);
+    }
+    if (ok) {
+      *reinterpret_cast<float4*>(J.A + int64_t(row) * R + 4 * q) = make_float4(o[0], o[1], o[2], o[3]);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        cmax[u] = fmaxf(cmax[u], fabsf(o[u]));
+        inner += double(Ms[rr][4 * q + u]) * double(o[u]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) As[rr][4 * q + u] = ok ? o[u] : 0.f;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+      const int e = int(threadIdx.x) + 256 * k;
+      if (e < R * R) {
+        const int a = e / R, b = e % R;
+        float sf = 0.f;
+#pragma unroll 8
+        for (int r = 0; r < TR; ++r) sf = fmaf(As[r][a], As[r][b], sf);
+        g[k] += double(sf);
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const int e = int(threadIdx.x) + 256 * k;
+    if (e < R * R) pg[int64_t(blockIdx.x) * R * R + e] = g[k];
+  }
+  // column max over the block: per column, the TR row threads in order
   __syncthreads();
+#pragma unroll
+  for (int u = 0; u < 4; ++u) As[rr][4 * q + u] = cmax[u];
+  red[threadIdx.x] = inner;
+  __syncthreads();
+  if (threadIdx.x < R) {
+    float mx = 0.f;
+    for (int r = 0; r < TR; ++r) mx = fmaxf(mx, As[r][threadIdx.x]);
+    pm[int64_t(blockIdx.x) * R + threadIdx.x] = double(mx);
+  }
   if (threadIdx.x == 0) {
+    double sacc = 0.0;
+    for (int t = 0; t < 256; ++t) sacc += red[t];
+    pi[blockIdx.x] = sacc;
+  }
+}
+
+// update_kernel for R = 32 on the matrix cores. Each wave takes 32-row tiles
+// (tile w + 4k of the block); per tile:
+//   A = M V^-1 as 16 v_mfma_f32_32x32x2f32 steps (operands from LDS: M tile
+//   and V^-1), then A^T A as 16 more on the A tile staged in LDS; the Gram
+//   accumulators are added into FP64 registers after every tile.
+// Accumulator layout of the 32x32 result: lane l holds column l % 32 and,
+// in register r, row (r % 4) + 4 * (l / 32) + 8 * (r / 4). Operand layout:
+// lane l supplies A[l % 32][k = l / 32] and B[k = l / 32][l % 32] for the
+// step's two k values. The loop is uniform over the block (idle tiles are
+// zero rows), so its barriers are reached by every wave.
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__global__ __launch_bounds__(256) void update32_mfma_kernel(const CpdJob* __restrict__ jobs,
+                                                            const int32_t* __restrict__ ublk0,
+                                                            int njobs, double* __restrict__ pg,
+                                                            double* __restrict__ pm,
+                                                            double* __restrict__ pi) {
+  constexpr int R = 32;
+  constexpr int W = 4;  // waves per block
+  __shared__ float Vs[R][R + 1];
+  __shared__ float Ts[W][32][R + 1];  // per wave: the M tile, then the A tile
+  __shared__ double Gs[R][R];         // cross-wave Gram reduction
+  __shared__ float Cm[W][64];
+  __shared__ double Is[W][64];
+  const int j = find_job(ublk0, njobs, blockIdx.x);
+  const CpdJob& J = jobs[j];
+  const int32_t r0 = (int32_t(blockIdx.x) - J.ublk0) * kUpdRows;
+  const int32_t r1 = min(r0 + kUpdRows, J.rows);
+  const int lane = int(threadIdx.x) & 63;
+  const int w = int(threadIdx.x) >> 6;
+  const int col = lane & 31, half = lane >> 5;
+  for (int e = threadIdx.x; e < R * R; e += 256) Vs[e / R][e % R] = J.Vinv[e];
+  double gd[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) gd[r] = 0.0;
+  float cmax = 0.f;
+  double inner = 0.0;
+  const int32_t ntiles = (r1 - r0 + 31) / 32;
+  const int32_t rounds = (ntiles + W - 1) / W;
+  for (int32_t k = 0; k < rounds; ++k) {
+    const int32_t t0 = r0 + (k * W + w) * 32;  // this wave's tile
+    __syncthreads();  // previous readers of Ts are done (and Vs is loaded)
+    // M tile -> LDS: 32 rows x 32 floats, float4 per lane, 4 passes
+#pragma unroll
+    for (int pss = 0; pss < 4; ++pss) {
+      const int e = pss * 64 + lane;  // float4 index in the tile
+      const int rr = e >> 3, q = e & 7;
+      float4 m4 = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (t0 + rr < r1) m4 = *reinterpret_cast<const float4*>(J.M + int64_t(t0 + rr) * R + 4 * q);
+      Ts[w][rr][4 * q] = m4.x;
+      Ts[w][rr][4 * q + 1] = m4.y;
+      Ts[w][rr][4 * q + 2] = m4.z;
+      Ts[w][rr][4 * q + 3] = m4.w;
+    }
+    __syncthreads();
+    f32x16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+#pragma unroll
+    for (int s = 0; s < R / 2; ++s) {
+      const float a = Ts[w][col][2 * s + half];
+      const float b = Vs[2 * s + half][col];
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+    }
+    // store A, column max, <M, A>; then stage A for the Gram
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int i = (r & 3) + 4 * half + 8 * (r >> 2);
+      if (t0 + i < r1) {
+        J.A[int64_t(t0 + i) * R + col] = acc[r];
+        cmax = fmaxf(cmax, fabsf(acc[r]));
+        inner += double(Ts[w][i][col]) * double(acc[r]);
+      }
+    }
+    __syncthreads();  // every lane has read its M values
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int i = (r & 3) + 4 * half + 8 * (r >> 2);
+      Ts[w][i][col] = (t0 + i < r1) ? acc[r] : 0.f;
+    }
+    __syncthreads();
+    f32x16 g;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) g[r] = 0.f;
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      const float v = Ts[w][2 * s + half][col];
+      g = __builtin_amdgcn_mfma_f32_32x32x2f32(v, v, g, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) gd[r] += double(g[r]);
+  }
+  // the block's partials, waves summed in order
+  for (int ww = 0; ww < W; ++ww) {
+    __syncthreads();
+    if (w == ww) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int i = (r & 3) + 4 * half + 8 * (r >> 2);
+        Gs[i][col] = (ww == 0 ? 0.0 : Gs[i][col]) + gd[r];
+      }
+    }
+  }
+  Cm[w][lane] = cmax;
+  Is[w][lane] = inner;
+  __syncthreads();
+  for (int e = threadIdx.x; e < R * R; e += 256) pg[int64_t(blockIdx.x) * R * R + e] = Gs[e / R][e % R];
+  if (threadIdx.x < R) {
+    float mx = 0.f;
+    for (int ww = 0; ww < W; ++ww) mx = fmaxf(mx, fmaxf(Cm[ww][threadIdx.x], Cm[ww][threadIdx.x + 32]));
+    pm[int64_t(blockIdx.x) * R + threadIdx.x] = double(mx);
+  }
+  if (threadIdx.x == 0) {
+    double sacc = 0.0;
+    for (int ww = 0; ww < W; ++ww)
+      for (int l = 0; l < 64; ++l) sacc += Is[ww][l];
+    pi[blockIdx.x] = sacc;
+  }
+}
+
+// Per job and chunk of 256 Gram entries (grid = jobs x ceil(R*R / 256)):
+// lambda (2-norms from the Gram diagonal on the first sweep, else
+// max(1, max |a|)), the Gram of the normalised factor G = (A^T A) ./
+// (lambda lambda^T), and the fit's inner product; partials in block order.
+// Every block of a job computes the same lambda; the chunk-0 block stores it.
+__global__ __launch_bounds__(256) void finish_kernel(const CpdJob* __restrict__ jobs, int R,
+                                                     int use_max,
+                                                     const double* __restrict__ pg,
+                                                     const double* __restrict__ pm,
+                                                     const double* __restrict__ pi) {
+  __shared__ double lam[kMaxR];
+  const int chunks = (R * R + 255) / 256;
+  const int chunk = int(blockIdx.x) % chunks;
+  const CpdJob& J = jobs[blockIdx.x / chunks];
+  for (int c = threadIdx.x; c < R; c += blockDim.x) {
+    double sq = 0.0, mx = 0.0;
+    for (int b = 0; b < J.nublk; ++b) {
+      sq += pg[int64_t(J.ublk0 + b) * R * R + c * R + c];
+      mx = fmax(mx, pm[int64_t(J.ublk0 + b) * R + c]);
+    }
+    double l = use_max ? fmax(mx, 1.0) : sqrt(sq);
+    if (!(l > 0.0)) l = 1.0;
+    lam[c] = double(float(l));
+    if (chunk == 0) J.lam[c] = float(l);
+  }
+  __syncthreads();
+  const int e = chunk * 256 + int(threadIdx.x);
+  if (e < R * R) {
     double s = 0.0;
-    for (int t = 0; t < 256; ++t) s += red[t];
-    part[blockIdx.x] = s;
+    for (int b = 0; b < J.nublk; ++b) s += pg[int64_t(J.ublk0 + b) * R * R + e];
+    J.G[e] = float(s / (lam[e / R] * lam[e % R]));
+  }
+  if (chunk == 0 && threadIdx.x == 0 && J.inner) {
+    double s = 0.0;
+    for (int b = 0; b < J.nublk; ++b) s += pi[J.ublk0 + b];
+    *J.inner = s;
+  }
+}
+
+// A /= lambda over every job's rows, float4 at a time (kApplyRows rows per
+// block).
+__global__ __launch_bounds__(256) void scale4_kernel(const CpdJob* __restrict__ jobs,
+                                                     const int32_t* __restrict__ ablk0, int njobs,
+                                                     int R) {
+  const int j = find_job(ablk0, njobs, blockIdx.x);
+  const CpdJob& J = jobs[j];
+  const int32_t r0 = (int32_t(blockIdx.x) - J.ablk0) * kApplyRows;
+  const int nr = min(kApplyRows, J.rows - r0);
+  const int Q = R / 4;
+  float4* A4 = reinterpret_cast<float4*>(J.A + int64_t(r0) * R);
+  for (int e = threadIdx.x; e < nr * Q; e += blockDim.x) {
+    const int qq = e % Q;
+    float4 a = A4[e];
+    a.x /= J.lam[4 * qq];
+    a.y /= J.lam[4 * qq + 1];
+    a.z /= J.lam[4 * qq + 2];
+    a.w /= J.lam[4 * qq + 3];
+    A4[e] = a;
   }
 }
 
@@ -296,6 +483,7 @@ struct tentaco_cpd {
   std::vector<std::vector<float*>> A, G;                   // [t][mode]
   std::vector<float*> lam, M, Vinv;                        // [t]
   std::vector<double> xnorm2;                              // ||X_t||^2
+  double* d_inner = nullptr;                               // [t] <X_t, model>
   int sweeps = 0;
   // per mode step
   struct Step {
@@ -303,13 +491,15 @@ struct tentaco_cpd {
     tentaco_batch* batch = nullptr;
     bool captured = false;
     CpdJob* d_jobs = nullptr;
-    int32_t *d_ablk0 = nullptr, *d_sblk0 = nullptr;
-    int32_t nablk = 0, nsblk = 0;
+    int32_t *d_ablk0 = nullptr, *d_sblk0 = nullptr, *d_ublk0 = nullptr;
+    int32_t nablk = 0, nsblk = 0, nublk = 0;
     std::vector<CpdJob> jobs;
   } step[kMaxOrder];
   // jobs for the Grams of every factor (initialisation)
   double* part = nullptr;
   int64_t part_len = 0;
+  // partials of the fused update (Gram, column max, inner product)
+  double *upg = nullptr, *upm = nullptr, *upi = nullptr;
   hipStream_t own = nullptr;
 
   ~tentaco_cpd() {
@@ -318,6 +508,7 @@ struct tentaco_cpd {
       (void)hipFree(s.d_jobs);
       (void)hipFree(s.d_ablk0);
       (void)hipFree(s.d_sblk0);
+      (void)hipFree(s.d_ublk0);
     }
     for (auto& v : A)
       for (float* p : v) (void)hipFree(p);
@@ -327,6 +518,10 @@ struct tentaco_cpd {
     for (float* p : M) (void)hipFree(p);
     for (float* p : Vinv) (void)hipFree(p);
     (void)hipFree(part);
+    (void)hipFree(upg);
+    (void)hipFree(upm);
+    (void)hipFree(upi);
+    (void)hipFree(d_inner);
     if (own) (void)hipStreamDestroy(own);
   }
 };
@@ -338,11 +533,13 @@ namespace {
 // other modes' Grams.
 int build_jobs(tentaco_cpd* c, int m, const std::vector<int>& tensors, std::vector<CpdJob>& jobs,
                std::vector<int32_t>& ablk0, std::vector<int32_t>& sblk0, int32_t& nablk,
-               int32_t& nsblk) {
+               int32_t& nsblk, std::vector<int32_t>* ublk0 = nullptr, int32_t* nublk = nullptr) {
   jobs.clear();
   ablk0.clear();
   sblk0.clear();
+  if (ublk0) ublk0->clear();
   nablk = nsblk = 0;
+  int32_t nu = 0;
   for (int t : tensors) {
     CpdJob J;
     J.M = c->M[size_t(t)];
@@ -357,12 +554,19 @@ int build_jobs(tentaco_cpd* c, int m, const std::vector<int>& tensors, std::vect
     J.nablk = (J.rows + kApplyRows - 1) / kApplyRows;
     J.sblk0 = nsblk;
     J.nsblk = (J.rows + kStatRows - 1) / kStatRows;
+    J.ublk0 = nu;
+    J.nublk = (J.rows + kUpdRows - 1) / kUpdRows;
+    // the fit needs <X, model> from each tensor's last mode update
+    if (c->d_inner && m == c->order[size_t(t)] - 1) J.inner = c->d_inner + t;
     ablk0.push_back(J.ablk0);
     sblk0.push_back(J.sblk0);
+    if (ublk0) ublk0->push_back(J.ublk0);
     nablk += J.nablk;
     nsblk += J.nsblk;
+    nu += J.nublk;
     jobs.push_back(J);
   }
+  if (nublk) *nublk = nu;
   return TENTACO_OK;
 }
 
@@ -515,6 +719,10 @@ int tentaco_cpd_create(tentaco_csf* const* handles, const int* level_modes, int 
     for (float v : vals) x2 += double(v) * double(v);
     c->xnorm2.push_back(x2);
   }
+  {
+    int rc = dalloc(&c->d_inner, ntensors);
+    if (rc) return bail(rc);
+  }
   // mode steps: one batch and one job table each
   for (int m = 0; m < kMaxOrder; ++m) {
     auto& S = c->step[m];
@@ -537,10 +745,20 @@ int tentaco_cpd_create(tentaco_csf* const* handles, const int* level_modes, int 
     if (!rc) rc = tentaco_batch_set_operands(S.batch, facs.data(), outs.data(), rows.data());
     if (!rc) rc = tentaco_batch_configure(S.batch, 1, 128, 64);
     if (rc) return bail(rc);
-    std::vector<int32_t> ab, sb;
-    build_jobs(c, m, S.tensors, S.jobs, ab, sb, S.nablk, S.nsblk);
+    std::vector<int32_t> ab, sb, ub;
+    build_jobs(c, m, S.tensors, S.jobs, ab, sb, S.nablk, S.nsblk, &ub, &S.nublk);
     rc = upload_jobs(S.jobs, ab, sb, &S.d_jobs, &S.d_ablk0, &S.d_sblk0);
+    if (!rc) rc = dalloc(&S.d_ublk0, int64_t(ub.size()));
     if (rc) return bail(rc);
+    TT_HIP_CHECK(hipMemcpy(S.d_ublk0, ub.data(), sizeof(int32_t) * ub.size(), hipMemcpyHostToDevice));
+  }
+  {
+    int64_t nu = 1;
+    for (auto& S : c->step) nu = std::max<int64_t>(nu, S.nublk);
+    int rc = 0;
+    if ((rc = dalloc(&c->upg, nu * R * R)) || (rc = dalloc(&c->upm, nu * R)) ||
+        (rc = dalloc(&c->upi, nu)))
+      return bail(rc);
   }
   // the reduction partials, sized once for the largest step (never
   // reallocated while kernels may still read them)
@@ -588,7 +806,6 @@ int tentaco_cpd_iterate(tentaco_cpd* c, int sweeps, int use_graphs, void* stream
   const int R = c->R;
   std::vector<double> inner(size_t(c->nt), 0.0);
   for (int it = 0; it < sweeps; ++it) {
-    const bool last = it == sweeps - 1;
     const int use_max = c->sweeps > 0 ? 1 : 0;
     for (int m = 0; m < kMaxOrder; ++m) {
       auto& S = c->step[m];
@@ -608,37 +825,31 @@ int tentaco_cpd_iterate(tentaco_cpd* c, int sweeps, int use_graphs, void* stream
       if (rc) return rc;
       hadamard_inverse_kernel<<<nj, 256, 0, st>>>(S.d_jobs, R);
       TT_HIP_CHECK(hipGetLastError());
-      apply_kernel<<<S.nablk, 256, 0, st>>>(S.d_jobs, S.d_ablk0, nj, R);
-      TT_HIP_CHECK(hipGetLastError());
-      colstat_kernel<<<S.nsblk, 256, 0, st>>>(S.d_jobs, S.d_sblk0, nj, R, use_max, c->part);
-      TT_HIP_CHECK(hipGetLastError());
-      lambda_kernel<<<nj, 64, 0, st>>>(S.d_jobs, R, use_max, c->part);
-      TT_HIP_CHECK(hipGetLastError());
-      scale_kernel<<<S.nablk, 256, 0, st>>>(S.d_jobs, S.d_ablk0, nj, R);
-      TT_HIP_CHECK(hipGetLastError());
-      rc = grams(c, S.d_jobs, S.d_sblk0, nj, S.nsblk, st);
-      if (rc) return rc;
-      if (last && fits) {
-        // <X, model> for the tensors whose last mode is m
-        inner_kernel<<<S.nsblk, 256, 0, st>>>(S.d_jobs, S.d_sblk0, nj, R, c->part);
-        TT_HIP_CHECK(hipGetLastError());
-        std::vector<double> p(size_t(S.nsblk));
-        TT_HIP_CHECK(hipMemcpyAsync(p.data(), c->part, sizeof(double) * p.size(),
-                                    hipMemcpyDeviceToHost, st));
-        TT_HIP_CHECK(hipStreamSynchronize(st));
-        for (size_t k = 0; k < S.jobs.size(); ++k) {
-          const int t = S.tensors[k];
-          if (c->order[size_t(t)] - 1 != m) continue;
-          double s = 0.0;
-          for (int b = 0; b < S.jobs[k].nsblk; ++b) s += p[size_t(S.jobs[k].sblk0 + b)];
-          inner[size_t(t)] = s;
-        }
+      switch (R) {
+        case 16:
+          update_kernel<16><<<S.nublk, 256, 0, st>>>(S.d_jobs, S.d_ublk0, nj, c->upg, c->upm, c->upi);
+          break;
+        case 32:
+          update32_mfma_kernel<<<S.nublk, 256, 0, st>>>(S.d_jobs, S.d_ublk0, nj, c->upg, c->upm,
+                                                        c->upi);
+          break;
+        default:
+          update_kernel<64><<<S.nublk, 256, 0, st>>>(S.d_jobs, S.d_ublk0, nj, c->upg, c->upm, c->upi);
+          break;
       }
+      TT_HIP_CHECK(hipGetLastError());
+      finish_kernel<<<nj * ((R * R + 255) / 256), 256, 0, st>>>(S.d_jobs, R, use_max, c->upg,
+                                                                 c->upm, c->upi);
+      TT_HIP_CHECK(hipGetLastError());
+      scale4_kernel<<<S.nablk, 256, 0, st>>>(S.d_jobs, S.d_ablk0, nj, R);
+      TT_HIP_CHECK(hipGetLastError());
     }
     ++c->sweeps;
   }
   TT_HIP_CHECK(hipStreamSynchronize(st));
   if (fits) {
+    TT_HIP_CHECK(hipMemcpy(inner.data(), c->d_inner, sizeof(double) * size_t(c->nt),
+                           hipMemcpyDeviceToHost));
     std::vector<float> g(static_cast<size_t>(R) * R), lam(static_cast<size_t>(R));
     for (int t = 0; t < c->nt; ++t) {
       std::vector<double> had(size_t(R) * R, 1.0);

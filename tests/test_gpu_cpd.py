@@ -28,11 +28,11 @@ def _tensor(rng, dims, density=0.6):
     return B, Xc.astype(np.float64)
 
 
-def _setup(seed, shapes, R):
+def _setup(seed, shapes, R, density=0.6):
     rng = np.random.default_rng(seed)
     hosts, dense, handles = [], [], []
     for dims in shapes:
-        B, X = _tensor(rng, dims)
+        B, X = _tensor(rng, dims, density)
         hosts.append(B)
         dense.append(X)
         handles.append([tt.DeviceCsf(Bm) for Bm in B.mode_csfs()])
@@ -73,6 +73,34 @@ def test_cpd_matches_oracle(gpu, use_graphs):
             assert _rel(a, A[m]) <= 2e-3, (t, m, _rel(a, A[m]))
         assert fits5[t] == pytest.approx(fits[-1], abs=1e-3)
         assert fits5[t] >= fits1[t] - 1e-4  # ALS does not lose fit
+    c.close()
+
+
+@pytest.mark.parametrize("R,shapes,sweeps", [
+    # a 5000-row mode: several 4096-row update blocks and every wave's tiles
+    (32, [(5000, 72, 68), (96, 70, 40, 36)], 5),
+    (64, [(4200, 72, 70)], 1),
+])
+def test_cpd_matches_oracle_ranks(gpu, R, shapes, sweeps):
+    """The R = 32 update on the matrix cores and the R = 64 update kernel,
+    against the FP64 oracle from the same initial factors."""
+    c, dense, inits, _ = _setup(11 + R, shapes, R, density=0.02)
+    fits1 = c.iterate(1, use_graphs=True)
+    for t, X in enumerate(dense):
+        A, lam, fits = ocpd.cp_als(X, inits[t], 1)
+        for m in range(X.ndim):
+            a, l = c.factor(t, m)
+            assert _rel(a, A[m]) <= 1e-4, (R, t, m, _rel(a, A[m]))
+        assert _rel(l, lam) <= 1e-4, (R, t)
+        assert fits1[t] == pytest.approx(fits[-1], abs=1e-4)
+    if sweeps > 1:
+        fitsn = c.iterate(sweeps - 1, use_graphs=True)
+        for t, X in enumerate(dense):
+            A, lam, fits = ocpd.cp_als(X, inits[t], sweeps)
+            for m in range(X.ndim):
+                a, _ = c.factor(t, m)
+                assert _rel(a, A[m]) <= 2e-3, (R, t, m, _rel(a, A[m]))
+            assert fitsn[t] == pytest.approx(fits[-1], abs=1e-3)
     c.close()
 
 
