@@ -335,6 +335,10 @@ int tentaco_mlp_train(tentaco_mlp* m, const float* x, const float* t, const floa
  * pred_ms (optional, n_out entries) = exp of the predictions. */
 int tentaco_mlp_select(tentaco_mlp* m, int order, int R, const float* features,
                        int* variant, float* margin, float* pred_ms);
+/* The same pick without margin or predictions: the variant-selection entry
+ * point of the handle API (features from tentaco_csf_features). */
+int tentaco_select_variant(tentaco_mlp* m, const float* features, int order, int R,
+                           int* variant);
 /* Attach a selector (not owned; NULL detaches) to a handle: AUTO on a rank
  * with no recorded choice then takes the selector's pick when its margin is
  * at least min_margin (0.05 = 5%) and records it (source SELECTOR); below

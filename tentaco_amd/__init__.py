@@ -143,6 +143,7 @@ def lib() -> ctypes.CDLL:
                                       c_int, c_int, c_float, ctypes.c_uint64, POINTER(c_float)]),
         "tentaco_mlp_select": (c_int, [c_void_p, c_int, c_int, POINTER(c_float), POINTER(c_int),
                                        POINTER(c_float), POINTER(c_float)]),
+        "tentaco_select_variant": (c_int, [c_void_p, POINTER(c_float), c_int, c_int, POINTER(c_int)]),
         "tentaco_batch_create": (c_int, [POINTER(c_void_p), c_int, c_int, POINTER(c_void_p)]),
         "tentaco_batch_set_operands": (c_int, [c_void_p, POINTER(c_void_p), POINTER(c_void_p), POINTER(c_int32)]),
         "tentaco_batch_configure": (c_int, [c_void_p, c_int, c_int, c_int]),

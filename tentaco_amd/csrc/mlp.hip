@@ -602,6 +602,11 @@ int tentaco_mlp_select(tentaco_mlp* m, int order, int R, const float* features, 
   return mlp_select(m, order, R, features, variant, margin, pred_ms);
 }
 
+int tentaco_select_variant(tentaco_mlp* m, const float* features, int order, int R,
+                           int* variant) {
+  return tentaco_mlp_select(m, order, R, features, variant, nullptr, nullptr);
+}
+
 int tentaco_csf_features(const tentaco_csf* h, int R, float* features) {
   clear_error();
   if (!h || !features) return fail(TENTACO_EINVAL, "features: NULL argument");

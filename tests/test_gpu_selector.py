@@ -141,6 +141,13 @@ def test_auto_uses_selector_pick_or_sweep(gpu):
     sel.set_params(p)
     v, margin, pred = sel.select(3, R, np.zeros(24, np.float32))
     assert v == want and margin == pytest.approx(np.e - 1, rel=1e-5)
+    # the plain selection entry point gives the same pick
+    import ctypes
+    vv = ctypes.c_int(-1)
+    f0 = np.zeros(24, np.float32)
+    assert tt.lib().tentaco_select_variant(sel.h, f0.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), 3, R,
+                                           ctypes.byref(vv)) == 0
+    assert vv.value == want
     H = tt.DeviceCsf(B)
     H.set_selector(sel, 0.05)
     H.mttkrp(fd, A, R, tt.VARIANT_AUTO)
