@@ -375,8 +375,8 @@ int tentaco_mttkrp_level_dev(tentaco_csf* h, int level, int R,
  * float per level N-2 node (nodes[N-2]). The first call on a handle (outside
  * stream capture) builds its TTV plan (a fibre-start bitmap, about nnz/8
  * bytes, and the window of c to cache in LDS) and synchronizes; later calls
- * allocate nothing and, when every fibre is non-empty, write each entry of
- * A_dev once without atomics (bitwise reproducible). A call captured before
+ * allocate nothing and write each entry of A_dev once without atomics
+ * (bitwise reproducible; empty fibres are written as zero). A call captured before
  * the plan exists uses the bracket kernel, which zero-fills A_dev and adds
  * the pieces of fibres that cross workgroups atomically. */
 int tentaco_ttv_dev(tentaco_csf* h, const float* c_dev, float* A_dev,

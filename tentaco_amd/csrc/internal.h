@@ -92,8 +92,15 @@ int launch_ttv(const CsfDev& B, const float* c, float* y, void* ws,
 // cross their ranges, and the window of c cached in LDS.
 struct TtvPlan {
   bool built = false;
-  bool ok = false;             // every fibre is non-empty: the unit kernel applies
+  bool ok = false;             // the unit kernel applies
   void* mem = nullptr;         // one allocation holding the arrays below
+  // with empty fibres, the bitmap and unit table index the non-empty fibres
+  // only: ne_fid maps them to output positions and empty_ids lists the
+  // fibres written as zero (both NULL without empty fibres; in mem2)
+  void* mem2 = nullptr;
+  int32_t* ne_fid = nullptr;
+  int32_t* empty_ids = nullptr;
+  int32_t n_empty = 0;
   uint32_t* flags = nullptr;   // nunits * 16 words
   int32_t* unit_fid = nullptr; // nunits entries
   int32_t* bnd_fid = nullptr;  // 2 per wave (max_waves)

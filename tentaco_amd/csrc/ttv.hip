@@ -5,10 +5,10 @@
 //
 // Two kernels. The unit kernel (below, ttv_unit_kernel) is the main path: it
 // reads a per-handle fibre-start bitmap instead of searching pos, keeps a
-// window of c in LDS, writes every y entry exactly once (no zero fill, no
-// atomics) and needs every fibre to be non-empty. The bracket kernel
-// (ttv_kernel) covers the rest (CSFs with empty fibres, e.g. a 2-mode CSF
-// with empty rows, and calls captured before a handle's plan exists):
+// window of c in LDS and writes every y entry exactly once (no zero fill, no
+// atomics); with empty fibres its plan indexes the non-empty ones and maps
+// them to output positions. The bracket kernel (ttv_kernel) covers calls
+// captured before a handle's plan exists and c beyond 2^30 entries:
 //
 // Work split: each workgroup owns 8*BS consecutive nonzeros, each lane 8
 // consecutive ones (two dwordx4 loads of crd and of vals per lane, fully
@@ -234,7 +234,8 @@ __global__ __launch_bounds__(kUnitThreads) void ttv_unit_kernel(
     const int32_t* __restrict__ crd, const float* __restrict__ vals,
     const uint32_t* __restrict__ flags, const int32_t* __restrict__ unit_fid,
     const float* __restrict__ cv, int32_t cdim, int32_t k0, int32_t lc, float* __restrict__ y,
-    int32_t nnz, int32_t nunits, int32_t* __restrict__ bnd_fid, float* __restrict__ bnd_val) {
+    int32_t nnz, int32_t nunits, int32_t* __restrict__ bnd_fid, float* __restrict__ bnd_val,
+    const int32_t* __restrict__ ne_fid) {
   __shared__ float s_c[LC];
   // c through a buffer descriptor: lanes served from LDS pass an offset past
   // the end, which returns 0 without a memory access, so the LDS and global
@@ -275,12 +276,14 @@ __global__ __launch_bounds__(kUnitThreads) void ttv_unit_kernel(
       hfid = cfid;
     }
   }
+  // output position of (non-empty) fibre f
+  auto yat = [&](int32_t f) -> float& { return y[ne_fid ? ne_fid[f] : f]; };
   auto emit = [&](int32_t fid, float val) {
     if (fid == hfid) {
       hval = val;
       hset = true;
     } else {
-      y[fid] = val;
+      yat(fid) = val;
     }
   };
   for (int32_t u = ub; u < ue; ++u) {
@@ -329,7 +332,7 @@ __global__ __launch_bounds__(kUnitThreads) void ttv_unit_kernel(
     for (int j = 1; j < 8; ++j) {
       sum += x[j - 1];
       if ((fb >> j) & 1u) {
-        if (closed) y[f] = sum;  // began and ended inside this lane
+        if (closed) yat(f) = sum;  // began and ended inside this lane
         else head = sum;
         closed = true;
         ++f;
@@ -374,8 +377,10 @@ __global__ __launch_bounds__(kUnitThreads) void ttv_unit_kernel(
 // workgroups on different XCDs)
 __global__ __launch_bounds__(256) void ttv_fold_kernel(
     const int32_t* __restrict__ bnd_fid, const float* __restrict__ bnd_val, int32_t E,
-    float* __restrict__ y) {
+    float* __restrict__ y, const int32_t* __restrict__ ne_fid,
+    const int32_t* __restrict__ empty_ids, int32_t n_empty) {
   const int32_t e = int32_t(blockIdx.x) * 256 + int32_t(threadIdx.x);
+  if (e < n_empty) y[empty_ids[e]] = 0.f;  // empty fibres
   if (e >= E) return;
   // the entry and its neighbours in one round of loads (runs are short:
   // a fibre has at most one entry per wave range it touches, and invalid
@@ -410,7 +415,7 @@ __global__ __launch_bounds__(256) void ttv_fold_kernel(
       }
     }
   }
-  y[f] = s;
+  y[ne_fid ? ne_fid[f] : f] = s;
 }
 
 }  // namespace
@@ -457,11 +462,13 @@ int launch_units_typed(const CsfDev& B, const TtvPlan& P, const float* c, float*
   ktimer_begin(st);
   ttv_unit_kernel<LC><<<P.grid, kUnitThreads, 0, st>>>(
       B.crd[B.order - 1], B.vals, P.flags, P.unit_fid, c, P.cdim, P.k0, P.lc, y, B.nnz, P.nunits,
-      P.bnd_fid, P.bnd_val);
+      P.bnd_fid, P.bnd_val, P.ne_fid);
   TT_HIP_CHECK(hipGetLastError());
   ktimer_end(st, "ttv_unit_kernel");
   const int32_t E = 2 * P.grid * kUnitWaves;
-  ttv_fold_kernel<<<(E + 255) / 256, 256, 0, st>>>(P.bnd_fid, P.bnd_val, E, y);
+  const int32_t nth = std::max(E, P.n_empty);
+  ttv_fold_kernel<<<(nth + 255) / 256, 256, 0, st>>>(P.bnd_fid, P.bnd_val, E, y, P.ne_fid,
+                                                       P.empty_ids, P.n_empty);
   TT_HIP_CHECK(hipGetLastError());
   return TENTACO_OK;
 }
@@ -471,6 +478,7 @@ int launch_units_typed(const CsfDev& B, const TtvPlan& P, const float* c, float*
 void free_ttv_plan(TtvPlan* P) {
   if (!P) return;
   (void)hipFree(P->mem);
+  (void)hipFree(P->mem2);
   *P = TtvPlan{};
 }
 
@@ -566,7 +574,52 @@ int build_ttv_plan(const CsfDev& B, int32_t cdim, TtvPlan* P, hipStream_t st) {
   uint32_t bad_h = 0;
   TT_HIP_CHECK(hipMemcpyAsync(&bad_h, bad, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   TT_HIP_CHECK(hipStreamSynchronize(st));
-  P->ok = bad_h == 0;
+  if (bad_h != 0) {
+    // empty fibres: rebuild the bitmap and unit table over the non-empty
+    // fibres (their starts are distinct), with the map back to output
+    // positions and the list of fibres to write as zero
+    std::vector<int32_t> pos(static_cast<size_t>(nf) + 1);
+    TT_HIP_CHECK(hipMemcpy(pos.data(), B.pos[B.order - 1], sizeof(int32_t) * pos.size(),
+                           hipMemcpyDeviceToHost));
+    std::vector<int32_t> pos_ne, ne, empty;
+    for (int32_t f = 0; f < nf; ++f) {
+      if (pos[size_t(f)] < pos[size_t(f) + 1]) {
+        pos_ne.push_back(pos[size_t(f)]);
+        ne.push_back(f);
+      } else if (pos[size_t(f)] == pos[size_t(f) + 1]) {
+        empty.push_back(f);
+      } else {
+        return TENTACO_OK;  // decreasing bounds: not a CSF the kernel can read
+      }
+    }
+    pos_ne.push_back(pos[size_t(nf)]);
+    const int32_t nne = int32_t(ne.size());
+    const size_t b_pos = align256(pos_ne.size() * sizeof(int32_t));
+    const size_t b_ne = align256(ne.size() * sizeof(int32_t) + 4);
+    const size_t b_em = align256(empty.size() * sizeof(int32_t) + 4);
+    if (hipMalloc(&P->mem2, b_pos + b_ne + b_em) != hipSuccess) {
+      P->mem2 = nullptr;
+      return fail(TENTACO_ENOMEM, "TTV plan: hipMalloc of the non-empty fibre map failed");
+    }
+    char* b2 = static_cast<char*>(P->mem2);
+    int32_t* d_pos_ne = reinterpret_cast<int32_t*>(b2);
+    P->ne_fid = reinterpret_cast<int32_t*>(b2 + b_pos);
+    P->empty_ids = reinterpret_cast<int32_t*>(b2 + b_pos + b_ne);
+    P->n_empty = int32_t(empty.size());
+    TT_HIP_CHECK(hipMemcpy(d_pos_ne, pos_ne.data(), sizeof(int32_t) * pos_ne.size(),
+                           hipMemcpyHostToDevice));
+    TT_HIP_CHECK(hipMemcpy(P->ne_fid, ne.data(), sizeof(int32_t) * ne.size(), hipMemcpyHostToDevice));
+    if (!empty.empty())
+      TT_HIP_CHECK(hipMemcpy(P->empty_ids, empty.data(), sizeof(int32_t) * empty.size(),
+                             hipMemcpyHostToDevice));
+    TT_HIP_CHECK(hipMemsetAsync(P->flags, 0, words * sizeof(uint32_t), st));
+    int64_t g = (int64_t(nne) + 255) / 256;
+    g = std::max<int64_t>(1, std::min<int64_t>(g, 16384));
+    ttv_plan_kernel<<<int(g), 256, 0, st>>>(d_pos_ne, nne, P->flags, P->unit_fid, bad);
+    TT_HIP_CHECK(hipGetLastError());
+    TT_HIP_CHECK(hipStreamSynchronize(st));
+  }
+  P->ok = true;
   return TENTACO_OK;
 }
 

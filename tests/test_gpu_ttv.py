@@ -147,9 +147,9 @@ def test_ttv_unit_kernel_sizes(gpu, n):
     assert np.array_equal(ones, np.diff(B.pos[2]).astype(np.float32))
 
 
-def test_ttv_empty_fibres_use_the_bracket_kernel(gpu):
-    """A 2-mode CSF with empty rows (dense root) is outside the unit
-    kernel's plan; the bracket kernel writes zeros there."""
+def test_ttv_empty_fibres(gpu):
+    """A 2-mode CSF with empty rows (dense root): the unit kernel's plan runs
+    over the non-empty fibres and the empty ones are written as zero."""
     rng = np.random.default_rng(76)
     rows = np.array([0, 2, 2, 5, 9], np.int32)
     coords = np.stack([rows[rng.integers(0, rows.size, 3000)], rng.integers(0, 700, 3000)])
@@ -165,5 +165,36 @@ def test_ttv_empty_fibres_use_the_bracket_kernel(gpu):
     y = tt.DeviceArray.empty((12,))
     H.ttv(tt.DeviceArray.from_numpy(c), y)
     total, k, name = tt.ktimer_stop()
-    assert name == "ttv_kernel"
+    assert name == "ttv_unit_kernel"
+    assert oracle.rel_l2(y.numpy(), ref) <= REL_L2_TOL
     H.close()
+
+
+def test_ttv_empty_fibres_inside_long_runs(gpu):
+    """Empty fibres between fibres that cross units and wave ranges (a
+    3-mode CSF handed over with empty level-1 nodes, as a caller may)."""
+    rng = np.random.default_rng(77)
+    B = random_csf(rng, (40, 300, 5000), 120_000)
+    # insert an empty fibre after every third fibre of level 1
+    pos2 = np.asarray(B.pos[2])
+    crd1 = np.asarray(B.crd[1])
+    pos1 = np.asarray(B.pos[1])
+    nf = crd1.size
+    keep_pos, keep_crd, new_pos1 = [0], [], [0]
+    for i in range(B.dims[0]):
+        for f in range(pos1[i], pos1[i + 1]):
+            keep_crd.append(crd1[f])
+            keep_pos.append(pos2[f + 1])
+            if f % 3 == 0 and crd1[f] + 1 < B.dims[1] and (f + 1 == pos1[i + 1] or crd1[f + 1] > crd1[f] + 1):
+                keep_crd.append(crd1[f] + 1)   # an empty fibre with its own coordinate
+                keep_pos.append(pos2[f + 1])
+        new_pos1.append(len(keep_crd))
+    E = tt.Csf(3, B.dims, [B.pos[0], np.array(new_pos1, np.int32), np.array(keep_pos, np.int32)],
+               [B.crd[0], np.array(keep_crd, np.int32), B.crd[2]], B.vals, _owner=B._owner)
+    assert E.nodes(1) > nf
+    c = rng.uniform(-1, 1, 5000).astype(np.float32)
+    got = _ttv_dev(E, c)
+    ref = oracle.ttv(E, c, "taco", True)
+    assert oracle.rel_l2(got, ref) <= REL_L2_TOL
+    empty = np.flatnonzero(np.diff(np.array(keep_pos)) == 0)
+    assert empty.size > 0 and np.all(got[empty] == 0.0)
