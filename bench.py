@@ -111,6 +111,16 @@ def profile_step(args):
             bt.launch()
         tt.synchronize()
         return
+    if args.profile_step == "cpals":
+        # CP-ALS on the 64-tensor batch: tuned, one warm sweep, then K sweeps
+        tensors = metrics.workload_batch()
+        per_tensor = [[tt.DeviceCsf(Bm) for Bm in B.mode_csfs()] for B in tensors]
+        cp = tt.Cpd(per_tensor, 32, seed=1005)
+        cp.tune(reps=2)
+        cp.iterate(args.warmup, use_graphs=True)
+        cp.iterate(args.steps, use_graphs=True)
+        tt.synchronize()
+        return
     if args.profile_step == "copy":
         src = tt.DeviceArray.empty((COPY_BYTES // 4,))
         dst = tt.DeviceArray.empty((COPY_BYTES // 4,))
@@ -680,7 +690,7 @@ def main():
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 FETCH_SIZE/WRITE_SIZE passes")
     ap.add_argument("--profile-step", choices=["mttkrp3", "copy", "ttv", "mttkrp3_100m", "batch",
-                                                "pertask"],
+                                                "pertask", "cpals"],
                     default=None,
                     help=argparse.SUPPRESS)
     ap.add_argument("--variant", default=None, help=argparse.SUPPRESS)
