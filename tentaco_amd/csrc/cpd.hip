@@ -411,12 +411,24 @@ __global__ __launch_bounds__(256) void finish_kernel(const CpdJob* __restrict__ 
   const int chunks = (R * R + 255) / 256;
   const int chunk = int(blockIdx.x) % chunks;
   const CpdJob& J = jobs[blockIdx.x / chunks];
-  for (int c = threadIdx.x; c < R; c += blockDim.x) {
-    double sq = 0.0, mx = 0.0;
-    for (int b = 0; b < J.nublk; ++b) {
-      sq += pg[int64_t(J.ublk0 + b) * R * R + c * R + c];
-      mx = fmax(mx, pm[int64_t(J.ublk0 + b) * R + c]);
+  // sums in block order; eight partials are loaded ahead of their adds
+  auto ordered_sum = [&](const double* base, int64_t stride) {
+    double s = 0.0;
+    int b = 0;
+    for (; b + 8 <= J.nublk; b += 8) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = base[int64_t(J.ublk0 + b + u) * stride];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s += v[u];
     }
+    for (; b < J.nublk; ++b) s += base[int64_t(J.ublk0 + b) * stride];
+    return s;
+  };
+  for (int c = threadIdx.x; c < R; c += blockDim.x) {
+    const double sq = ordered_sum(pg + c * R + c, int64_t(R) * R);
+    double mx = 0.0;
+    for (int b = 0; b < J.nublk; ++b) mx = fmax(mx, pm[int64_t(J.ublk0 + b) * R + c]);
     double l = use_max ? fmax(mx, 1.0) : sqrt(sq);
     if (!(l > 0.0)) l = 1.0;
     lam[c] = double(float(l));
@@ -425,15 +437,10 @@ __global__ __launch_bounds__(256) void finish_kernel(const CpdJob* __restrict__ 
   __syncthreads();
   const int e = chunk * 256 + int(threadIdx.x);
   if (e < R * R) {
-    double s = 0.0;
-    for (int b = 0; b < J.nublk; ++b) s += pg[int64_t(J.ublk0 + b) * R * R + e];
+    const double s = ordered_sum(pg + e, int64_t(R) * R);
     J.G[e] = float(s / (lam[e / R] * lam[e % R]));
   }
-  if (chunk == 0 && threadIdx.x == 0 && J.inner) {
-    double s = 0.0;
-    for (int b = 0; b < J.nublk; ++b) s += pi[J.ublk0 + b];
-    *J.inner = s;
-  }
+  if (chunk == 0 && threadIdx.x == 0 && J.inner) *J.inner = ordered_sum(pi, 1);
 }
 
 // A /= lambda over every job's rows, float4 at a time (kApplyRows rows per
