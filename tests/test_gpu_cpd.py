@@ -140,3 +140,37 @@ def test_cpd_rejects_bad_input(gpu):
         tt.Cpd([hs], 24)  # R outside {16, 32, 64}
     with pytest.raises(tt.TentacoError):
         tt.Cpd([hs], 16, level_modes=[[[1, 0, 2], [1, 0, 2], [2, 0, 1]]])  # mode 0 not at its root
+
+
+def _coo_of(B):
+    paths = [np.arange(B.nodes(0))]
+    for m in range(1, B.order):
+        pos = np.asarray(B.pos[m])
+        parent = np.repeat(np.arange(B.nodes(m - 1)), np.diff(pos))
+        paths = [p[parent] for p in paths] + [np.arange(B.nodes(m))]
+    return (np.stack([np.asarray(B.crd[m])[paths[m]] for m in range(B.order)]),
+            np.asarray(B.vals, np.float64))
+
+
+def test_cpd_sparse_high_order_large_modes(gpu):
+    """5-mode tensors with modes of 2-6e4 rows and few nonzeros (the shape
+    of the batch's sparse order-5 tensors, where FP32 products of four
+    factors' scales are at risk), from the device's own seeded factors,
+    against the sparse FP64 oracle: fits and factors after 1 and 5 sweeps."""
+    dims_list = [(60_000, 50_000, 40_000, 30_000, 20_000), (45_000, 3_000, 52_000, 800, 38_000)]
+    tensors = [tt.Csf.synth(5, d, 15_000, slice_s=1.2, seed=31 + k) for k, d in enumerate(dims_list)]
+    c = tt.Cpd([[tt.DeviceCsf(Bm) for Bm in B.mode_csfs()] for B in tensors], 32, seed=1005)
+    inits = [[c.factor(t, m)[0].astype(np.float64) for m in range(5)] for t in range(len(tensors))]
+    f1 = c.iterate(1, use_graphs=True)
+    f5 = c.iterate(4, use_graphs=True)
+    for t, B in enumerate(tensors):
+        co, v = _coo_of(B)
+        A1, _, fits1 = ocpd.cp_als_sparse(co, v, B.dims, inits[t], 1)
+        A5, lam, fits5 = ocpd.cp_als_sparse(co, v, B.dims, inits[t], 5)
+        assert 0.0 <= f1[t] < 1.0 and f1[t] == pytest.approx(fits1[0], abs=1e-4), (t, f1[t], fits1)
+        assert f5[t] == pytest.approx(fits5[-1], abs=1e-3), (t, f5[t], fits5)
+        for m in range(5):
+            a, l = c.factor(t, m)
+            assert _rel(a, A5[m]) <= 2e-3, (t, m, _rel(a, A5[m]))
+        assert _rel(l, lam) <= 2e-3
+    c.close()

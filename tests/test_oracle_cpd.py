@@ -45,3 +45,24 @@ def test_fit_matches_explicit_reconstruction_and_mttkrp():
             for k in range(3):
                 ref[j] += X[i, j, k] * A[0][i] * A[2][k]
     assert np.allclose(M, ref)
+
+
+def test_sparse_als_matches_dense_als():
+    """The COO form of the oracle follows the dense one sweep by sweep
+    (factors, lambda and the fit from its formula), on a 4-mode tensor with
+    empty slices and repeated coordinates summed."""
+    rng = np.random.default_rng(21)
+    dims = (9, 8, 7, 6)
+    n = 700
+    coords = np.stack([rng.integers(0, d, n) for d in dims])
+    vals = rng.uniform(-1, 1, n)
+    X = np.zeros(dims)
+    np.add.at(X, tuple(coords), vals)
+    R = 5
+    init = [rng.uniform(0, 1, (d, R)) for d in dims]
+    Ad, ld, fd = cpd.cp_als(X, init, 4)
+    As, ls, fs = cpd.cp_als_sparse(coords, vals, dims, init, 4)
+    for a, b in zip(Ad, As):
+        assert np.allclose(a, b, rtol=1e-10, atol=1e-12)
+    assert np.allclose(ld, ls, rtol=1e-10)
+    assert np.allclose(fd, fs, rtol=1e-9, atol=1e-12)
