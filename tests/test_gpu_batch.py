@@ -98,3 +98,31 @@ def test_batch_autotune_picks_a_valid_config(gpu):
     bt.run()
     for B, f, o in zip(hosts, facs_h, outs):
         assert oracle.rel_l2(o.numpy(), oracle.mttkrp(B, f, "taco", fp64=True)) <= REL_L2_TOL
+
+
+@pytest.mark.parametrize("R", [16, 64])
+def test_batch_other_ranks_eager_and_graph(gpu, R):
+    """R = 16 and 64 (four and sixteen lanes per factor row) through both
+    edge families, eager and replayed, against the oracle; every output is
+    poisoned before each run."""
+    tensors = _tensors(9 + R, n=4)
+    tasks, hosts, facs_h, facs_d, outs, rows = _setup(tensors, R, 10 + R)
+    refs = [oracle.mttkrp(B, f, "taco", fp64=True) for B, f in zip(hosts, facs_h)]
+    bt = tt.Batch(tasks, R)
+    bt.set_operands(facs_d, outs, rows)
+    for fam, blk, grp in ((0, 256, 8), (1, 128, 64)):
+        bt.configure(fam, blk, grp)
+        for o in outs:
+            o.upload(np.full(o.shape, 9.0, np.float32))
+        bt.run()
+        for t, (o, ref) in enumerate(zip(outs, refs)):
+            assert oracle.rel_l2(o.numpy(), ref) <= REL_L2_TOL, (R, fam, t)
+        bt.capture()
+        for rep in range(2):
+            for o in outs:
+                o.upload(np.full(o.shape, -4.0, np.float32))
+            bt.launch()
+            tt.synchronize()
+            for t, (o, ref) in enumerate(zip(outs, refs)):
+                assert oracle.rel_l2(o.numpy(), ref) <= REL_L2_TOL, (R, fam, rep, t)
+    bt.close()
