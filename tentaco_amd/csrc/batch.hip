@@ -21,6 +21,7 @@ struct tentaco_batch {
   int64_t* d_fill_off = nullptr;
   int64_t* d_bnd_off = nullptr;
   int64_t* d_item_off = nullptr;
+  uint32_t* d_rowbits = nullptr;  // per task, one bit per output row
   int32_t* d_counter = nullptr;
   int32_t* d_brk = nullptr;
   size_t brk_bytes = 0;
@@ -112,6 +113,7 @@ int configure(tentaco_batch* b, int family, int block, int group) {
   L.item_off = b->d_item_off;
   L.total_items = int32_t(item[b->ntasks]);
   L.counter = b->d_counter;
+  L.full_fill = family != kEdgeParallel;
   L.family = family;
   L.block = block;
   L.group = group;
@@ -204,12 +206,26 @@ int tentaco_batch_set_operands(tentaco_batch* b, const float* const* factors_dev
       T.F.M[m] = f;
     }
     T.A = outs_dev[t];
+    T.rows = out_rows[t];
     fill[s + 1] = fill[s] + int64_t(out_rows[t]) * b->R;
   }
   (void)hipFree(b->d_fill_off);
   b->d_fill_off = nullptr;
+  (void)hipFree(b->d_rowbits);
+  b->d_rowbits = nullptr;
+  // the row bitmaps of every task (one 32-bit word per 32 rows), built once
+  std::vector<int64_t> woff(b->ntasks + 1, 0);
+  for (int s = 0; s < b->ntasks; ++s) woff[s + 1] = woff[s] + (b->host_tasks[s].rows + 31) / 32;
   int rc = dev_alloc(&b->d_fill_off, size_t(b->ntasks));
+  if (!rc) rc = dev_alloc(&b->d_rowbits, size_t(woff[b->ntasks]));
   if (rc) return rc;
+  TT_HIP_CHECK(hipMemset(b->d_rowbits, 0, sizeof(uint32_t) * std::max<int64_t>(woff[b->ntasks], 1)));
+  for (int s = 0; s < b->ntasks; ++s) {
+    b->host_tasks[s].rowbits = b->d_rowbits + woff[s];
+    rc = launch_batch_rowbits(b->host_tasks[s].B, b->d_rowbits + woff[s], nullptr);
+    if (rc) return rc;
+  }
+  TT_HIP_CHECK(hipDeviceSynchronize());
   TT_HIP_CHECK(hipMemcpy(b->d_fill_off, fill.data(), sizeof(int64_t) * b->ntasks,
                          hipMemcpyHostToDevice));
   b->L.fill_off = b->d_fill_off;
@@ -351,6 +367,7 @@ int tentaco_batch_free(tentaco_batch* b) {
   free_tables(b);
   (void)hipFree(b->d_counter);
   (void)hipFree(b->d_fill_off);
+  (void)hipFree(b->d_rowbits);
   for (int k = 0; k <= kMaxOrder; ++k)
     if (b->ev[k]) (void)hipEventDestroy(b->ev[k]);
   if (b->side) (void)hipStreamDestroy(b->side);

@@ -131,6 +131,10 @@ struct BatchTaskDev {
   FactorsDev F;
   float* A = nullptr;
   int32_t* brk = nullptr;
+  // output rows, and one bit per row, set where a root node with at least
+  // one nonzero writes the row
+  int32_t rows = 0;
+  const uint32_t* rowbits = nullptr;
 };
 
 struct BatchLaunch {
@@ -143,6 +147,11 @@ struct BatchLaunch {
   const int64_t* item_off = nullptr;    // device, prefix of items per task (ntasks + 1)
   int32_t total_items = 0;
   int32_t* counter = nullptr;           // device work counter
+  // false for the edge-parallel family: the prep then skips the output rows
+  // that root nodes with nonzeros store (their bits in the tasks' rowbits),
+  // except the rows of the slices holding an item's first or last nonzero,
+  // which take atomics. true: whole outputs are zero-filled.
+  bool full_fill = true;
   int family = 1, block = 256, group = 16, R = 32;
   int schedule = 0;  // 0: device work counter, 1: static cyclic items
   int32_t NB = 0;
@@ -167,6 +176,9 @@ struct BatchLaunch {
 };
 
 int launch_batch(const BatchLaunch& L, hipStream_t stream);
+// Set, in bits (pre-cleared, one bit per output row), the row of every root
+// node of B that holds at least one nonzero.
+int launch_batch_rowbits(const CsfDev& B, uint32_t* bits, hipStream_t stream);
 int batch_occupancy(int family, int block, int R, int32_t NB, int order, int* per_cu);
 
 // Fill n floats with value on `stream` (kernel, not hipMemsetAsync, so it is

@@ -836,9 +836,27 @@ __device__ __forceinline__ int32_t upper_index(const int64_t* __restrict__ off, 
   return lo;
 }
 
+// Root node of CSF B that holds nonzero position p (the largest node at each
+// level whose range starts at or before the key, so empty nodes are skipped).
+__device__ __forceinline__ int32_t root_of_position(const CsfDev& B, int32_t p) {
+  int32_t key = p;
+  for (int m = B.order - 2; m >= 0; --m) key = search_le(B.pos[m + 1], 0, B.nodes[m] - 1, key);
+  return key;
+}
+
+__device__ __forceinline__ void zero_row(float* A, int64_t row, int R) {
+  float4* A4 = reinterpret_cast<float4*>(A + row * R);
+  for (int q = 0; q < (R >> 2); ++q) A4[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
 // Prep of the order group holding sorted tasks [t_lo, t_hi): reset that
-// order's work counter, zero the group's outputs (quads [q_lo, q_hi)) and
-// write its brackets (entries [b_lo, b_hi)).
+// order's work counter, write its brackets (entries [b_lo, b_hi)) and zero
+// the outputs (quads [q_lo, q_hi)). With full_fill every row is zeroed.
+// Otherwise (edge-parallel) a row whose bit is set in the task's rowbits is
+// skipped, because a root node with nonzeros stores it, unless it is the row
+// of a slice holding position b*NB or b*NB - 1 of some boundary b: those are
+// the walk's only atomically updated rows (the slices holding an item's first
+// and last nonzero) and are zeroed with the brackets.
 __global__ __launch_bounds__(256) void batch_prep_kernel(const BatchTaskDev* __restrict__ tasks,
                                                          int32_t t_lo, int32_t t_hi,
                                                          const int64_t* __restrict__ fill_off,
@@ -846,7 +864,8 @@ __global__ __launch_bounds__(256) void batch_prep_kernel(const BatchTaskDev* __r
                                                          const int64_t* __restrict__ bnd_off,
                                                          int64_t b_lo, int64_t b_hi,
                                                          int32_t* __restrict__ counter,
-                                                         int32_t NB) {
+                                                         int32_t NB, int full_fill, int R,
+                                                         int qshift) {
   const int64_t tid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t nthr = int64_t(gridDim.x) * blockDim.x;
   if (tid == 0) *counter = 0;
@@ -856,6 +875,7 @@ __global__ __launch_bounds__(256) void batch_prep_kernel(const BatchTaskDev* __r
   // checks). Each workgroup takes one contiguous range of quads, finds the
   // task of its first quad with one search and walks forward across task
   // boundaries (empty tasks included).
+  // (quad e of a task is in row e >> qshift, with R / 4 == 1 << qshift)
   const int64_t total_q = q_hi - q_lo;
   const int64_t per = (total_q + gridDim.x - 1) / gridDim.x;
   const int64_t q0 = q_lo + int64_t(blockIdx.x) * per;
@@ -865,6 +885,7 @@ __global__ __launch_bounds__(256) void batch_prep_kernel(const BatchTaskDev* __r
     int64_t base = fill_off[t] >> 2;
     int64_t next = t + 1 < t_hi ? fill_off[t + 1] >> 2 : q_hi;
     float4* A4 = reinterpret_cast<float4*>(tasks[t].A);
+    const uint32_t* bits = tasks[t].rowbits;
     for (int64_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
       if (q >= next) {
         do {
@@ -873,8 +894,14 @@ __global__ __launch_bounds__(256) void batch_prep_kernel(const BatchTaskDev* __r
         } while (q >= next);
         base = fill_off[t] >> 2;
         A4 = reinterpret_cast<float4*>(tasks[t].A);
+        bits = tasks[t].rowbits;
       }
-      A4[q - base] = make_float4(0.f, 0.f, 0.f, 0.f);
+      const int64_t e = q - base;
+      if (!full_fill) {
+        const int64_t r = e >> qshift;
+        if ((bits[r >> 5] >> (r & 31)) & 1u) continue;
+      }
+      A4[e] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
   }
   for (int64_t i = b_lo + tid; i < b_hi; i += nthr) {
@@ -890,6 +917,30 @@ __global__ __launch_bounds__(256) void batch_prep_kernel(const BatchTaskDev* __r
       const int32_t x = search_le(T.B.pos[m + 1], 0, T.B.nodes[m] - 1, key);
       T.brk[bi * (N - 1) + m] = x;
       key = x;
+    }
+    if (!full_fill) {
+      zero_row(T.A, T.B.crd0[key], R);
+      int64_t q = bi * int64_t(NB) - 1;
+      if (q > T.B.nnz - 1) q = T.B.nnz - 1;
+      if (bi > 0 && q != p) {
+        const int32_t x1 = root_of_position(T.B, int32_t(q));
+        if (x1 != key) zero_row(T.A, T.B.crd0[x1], R);
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void batch_rowbits_kernel(CsfDev B, uint32_t* __restrict__ bits) {
+  const int64_t nthr = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t x = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; x < B.nodes[0]; x += nthr) {
+    int32_t lo = int32_t(x), hi = int32_t(x) + 1;
+    for (int m = 1; m < B.order; ++m) {
+      lo = B.pos[m][lo];
+      hi = B.pos[m][hi];
+    }
+    if (hi > lo) {
+      const int32_t r = B.crd0[x];
+      atomicOr(bits + (r >> 5), 1u << (r & 31));
     }
   }
 }
@@ -965,11 +1016,14 @@ int launch_batch_prep(const BatchLaunch& L, int N, hipStream_t st) {
   const int64_t f0 = L.h_fill_off[t_lo], f1 = L.h_fill_off[t_hi];
   const int64_t b0 = L.h_bnd_off[t_lo], b1 = L.h_bnd_off[t_hi];
   const int64_t work = std::max<int64_t>((f1 - f0) >> 2, b1 - b0);
+  const int qshift = L.R == 16 ? 2 : L.R == 32 ? 3 : 4;  // log2(R / 4), R in {16, 32, 64}
   batch_prep_kernel<<<grid_for(work, 256, 16384), 256, 0, st>>>(
-      L.tasks, t_lo, t_hi, L.fill_off, f0 >> 2, f1 >> 2, L.bnd_off, b0, b1, L.counter + N, L.NB);
+      L.tasks, t_lo, t_hi, L.fill_off, f0 >> 2, f1 >> 2, L.bnd_off, b0, b1, L.counter + N, L.NB,
+      L.full_fill ? 1 : 0, L.R, qshift);
   TT_HIP_CHECK(hipGetLastError());
   return TENTACO_OK;
 }
+
 
 template <int R>
 int launch_batch_typed(const BatchLaunch& L, hipStream_t st) {
@@ -1176,6 +1230,13 @@ int launch_expand3(const CsfDev& B, int32_t* kq, int32_t* nzr, hipStream_t st) {
 int launch_fill(float* p, int64_t n, float value, hipStream_t st) {
   if (n <= 0) return TENTACO_OK;
   fill_kernel<<<grid_for(n / 4 + 1, 256, 16384), 256, 0, st>>>(p, n, value);
+  TT_HIP_CHECK(hipGetLastError());
+  return TENTACO_OK;
+}
+
+int launch_batch_rowbits(const CsfDev& B, uint32_t* bits, hipStream_t st) {
+  if (B.nodes[0] <= 0) return TENTACO_OK;
+  batch_rowbits_kernel<<<grid_for(B.nodes[0], 256, 8192), 256, 0, st>>>(B, bits);
   TT_HIP_CHECK(hipGetLastError());
   return TENTACO_OK;
 }

@@ -7,7 +7,7 @@ import pytest
 
 import oracle
 import tentaco_amd as tt
-from helpers import REL_L2_TOL
+from helpers import REL_L2_TOL, factors_for
 
 pytestmark = pytest.mark.gpu
 
@@ -126,3 +126,105 @@ def test_batch_other_ranks_eager_and_graph(gpu, R):
             for t, (o, ref) in enumerate(zip(outs, refs)):
                 assert oracle.rel_l2(o.numpy(), ref) <= REL_L2_TOL, (R, fam, rep, t)
     bt.close()
+
+
+def _with_empty_fibres(B):
+    """The 3-mode CSF B with an empty level-1 fibre under every root node
+    that had no children (so that node has children but no nonzeros) and
+    after every fourth fibre where the next coordinate is free."""
+    pos1, crd1, pos2 = (np.asarray(a) for a in (B.pos[1], B.crd[1], B.pos[2]))
+    new_pos1, new_crd1, new_pos2 = [0], [], [0]
+    for i in range(B.dims[0]):
+        if pos1[i] == pos1[i + 1]:
+            new_crd1.append(0)
+            new_pos2.append(new_pos2[-1])
+        for f in range(pos1[i], pos1[i + 1]):
+            new_crd1.append(crd1[f])
+            new_pos2.append(pos2[f + 1])
+            if f % 4 == 0 and crd1[f] + 1 < B.dims[1] and (
+                    f + 1 == pos1[i + 1] or crd1[f + 1] > crd1[f] + 1):
+                new_crd1.append(crd1[f] + 1)
+                new_pos2.append(pos2[f + 1])
+        new_pos1.append(len(new_crd1))
+    return tt.Csf(3, B.dims, [B.pos[0], np.array(new_pos1, np.int32), np.array(new_pos2, np.int32)],
+                  [B.crd[0], np.array(new_crd1, np.int32), B.crd[2]], B.vals, _owner=B._owner)
+
+
+def test_batch_rows_without_nonzeros_are_zero(gpu):
+    """Outputs whose rows have no nonzeros: a compressed root with gaps, root
+    nodes whose only children are empty fibres, output rows beyond the
+    largest coordinate, and one Zipf tensor whose slices cross many items.
+    Such rows must read exactly zero after every run (the edge-parallel
+    family writes them from its zero items, the edge-serial family from the
+    prep's fill), with outputs poisoned before each run, eager and replayed."""
+    from helpers import random_coo
+    rng = np.random.default_rng(31)
+    R = 32
+    # compressed root: 40 of 3000 rows listed
+    coords, vals = random_coo(rng, (40, 60, 70), 6000)
+    listed = np.sort(rng.choice(3000, 40, replace=False)).astype(np.int32)
+    coords[0] = listed[coords[0]]
+    D = tt.Csf.from_coo((3000, 60, 70), coords, vals)
+    nonempty = np.flatnonzero(np.diff(D.pos[1])).astype(np.int32)
+    pos1 = np.append(D.pos[1][nonempty], D.pos[1][-1]).astype(np.int32)
+    Bc = tt.Csf(3, D.dims, [np.array([0, nonempty.size], np.int32), pos1, D.pos[2]],
+                [nonempty, D.crd[1], D.crd[2]], D.vals, _owner=D._owner)
+    # dense root, every third slice empty, then empty fibres inserted
+    coords, vals = random_coo(rng, (700, 90, 80), 30000)
+    coords[0] = (coords[0] // 3) * 3
+    E = _with_empty_fibres(tt.Csf.from_coo((700, 90, 80), coords, vals))
+    # a Zipf tensor with long slices, and a tiny one
+    Z = tt.Csf.synth(3, (5000, 400, 300), 200_000, slice_s=1.3, seed=32)
+    T = tt.Csf.from_coo((9, 5, 5), np.array([[3], [2], [4]], np.int32), np.array([0.5], np.float32))
+    hosts = [(Bc, D), (E, E), (Z, Z), (T, T)]
+    extra = [0, 0, 1234, 77]  # output rows beyond dims[0]
+    tasks, facs_h, facs_d, outs, rows, refs, zero_rows = [], [], [], [], [], [], []
+    for (B, Bref), x in zip(hosts, extra):
+        facs = factors_for(rng, B, R)
+        n = B.dims[0] + x
+        ref = np.zeros((n, R))
+        ref[:B.dims[0]] = oracle.mttkrp(Bref, facs, "taco", fp64=True)
+        hit = np.zeros(n, bool)
+        c0 = np.asarray(B.crd[0])
+        lo, hi = np.arange(c0.size), np.arange(c0.size) + 1
+        for m in range(1, B.order):
+            lo, hi = np.asarray(B.pos[m])[lo], np.asarray(B.pos[m])[hi]
+        hit[c0[hi > lo]] = True
+        tasks.append(tt.DeviceCsf(B))
+        facs_h.append(facs)
+        facs_d.append([tt.DeviceArray.from_numpy(f) for f in facs])
+        outs.append(tt.DeviceArray.empty((n, R)))
+        rows.append(n)
+        refs.append(ref)
+        zero_rows.append(np.flatnonzero(~hit))
+    assert all(z.size > 0 for z in zero_rows)
+    bt = tt.Batch(tasks, R)
+    bt.set_operands(facs_d, outs, rows)
+
+    def check(tag):
+        for t, (o, ref, zr) in enumerate(zip(outs, refs, zero_rows)):
+            got = o.numpy()
+            assert np.all(got[zr] == 0.0), (tag, t)
+            assert oracle.rel_l2(got, ref) <= REL_L2_TOL, (tag, t)
+
+    for sched in (0, 1):
+        for fam, blk, grp in ((1, 128, 64), (1, 256, 4), (1, 512, 16), (0, 256, 8)):
+            bt.configure(fam, blk, grp)
+            bt.set_schedule(sched)
+            for o in outs:
+                o.upload(np.full(o.shape, 9.0, np.float32))
+            bt.run()
+            tt.synchronize()
+            check((sched, fam, blk, grp))
+    bt.configure(1, 128, 64)
+    bt.set_schedule(0)
+    bt.capture()
+    for rep in range(3):
+        for o in outs:
+            o.upload(np.full(o.shape, float(rep) - 5.0, np.float32))
+        bt.launch()
+        tt.synchronize()
+        check(("graph", rep))
+    bt.close()
+    for H in tasks:
+        H.close()
