@@ -97,7 +97,9 @@ void tentaco_legacy_ttv3(const char* name, taco_tensor_t* A, taco_tensor_t* B,
                          taco_tensor_t* C);
 
 /* Milliseconds of the last call's hot region (partition + zero + kernel),
- * hipEvent-timed like GPUTimer (gpu_library.h:949-988); -1 before any call. */
+ * hipEvent-timed like GPUTimer (gpu_library.h:949-988); -1 before any call.
+ * Structures built once per CSF (the slice-row expansion, the TTV plan) are
+ * built before the region starts. */
 float get_gpu_timer_result(void);
 /* TACO schedule (mttkrp3_csf_gpu_taco.h:73); runs the fixed default variant. */
 void mttkrp_gpu_taco(taco_tensor_t* A, taco_tensor_t* B, taco_tensor_t* C,

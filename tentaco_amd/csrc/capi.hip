@@ -307,6 +307,10 @@ int host_mttkrp(int order, taco_tensor_t* A, taco_tensor_t* B,
   if (vr.family < 0) return fail(TENTACO_EINVAL, "variant %d out of range", variant);
   rc = reserve_ws(h, mttkrp_workspace_bytes(h->view(), R, vr));
   if (rc) return rc;
+  if (vr.family == kSliceRow) {
+    rc = ensure_expand3(h, nullptr);  // per-handle structure, outside the timed region
+    if (rc) return rc;
+  }
   TT_HIP_CHECK(hipDeviceSynchronize());
   {
     std::lock_guard<std::mutex> lk(g_timer_mu);
