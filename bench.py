@@ -687,8 +687,11 @@ def run_batch_workload(tt, metrics, K):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=25)
-    ap.add_argument("--warmup", type=int, default=5)
+    # 400 steps make the C1 region about 3 ms long: the start event fires on
+    # an idle GPU before the host has queued the first launch, and over 25
+    # steps that host latency alone adds about 2 us per step
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--quick", action="store_true", help="skip the 100M-nnz and TTV extra lines")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-traffic", action="store_true",
