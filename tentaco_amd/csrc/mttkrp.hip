@@ -886,22 +886,40 @@ __global__ __launch_bounds__(256) void batch_prep_kernel(const BatchTaskDev* __r
     int64_t next = t + 1 < t_hi ? fill_off[t + 1] >> 2 : q_hi;
     float4* A4 = reinterpret_cast<float4*>(tasks[t].A);
     const uint32_t* bits = tasks[t].rowbits;
-    for (int64_t q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
-      if (q >= next) {
-        do {
-          ++t;
-          next = t + 1 < t_hi ? fill_off[t + 1] >> 2 : q_hi;
-        } while (q >= next);
-        base = fill_off[t] >> 2;
-        A4 = reinterpret_cast<float4*>(tasks[t].A);
-        bits = tasks[t].rowbits;
-      }
-      const int64_t e = q - base;
-      if (!full_fill) {
+    // four quads per thread and pass, their bitmap words loaded before any
+    // of their stores
+    constexpr int U = 4;
+    const int64_t stride = blockDim.x;
+    for (int64_t q = q0 + threadIdx.x; q < q1; q += U * stride) {
+      float4* dst[U];
+      const uint32_t* wp[U];
+      int sh[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t qq = q + u * stride;
+        dst[u] = nullptr;
+        if (qq >= q1) continue;
+        if (qq >= next) {
+          do {
+            ++t;
+            next = t + 1 < t_hi ? fill_off[t + 1] >> 2 : q_hi;
+          } while (qq >= next);
+          base = fill_off[t] >> 2;
+          A4 = reinterpret_cast<float4*>(tasks[t].A);
+          bits = tasks[t].rowbits;
+        }
+        const int64_t e = qq - base;
         const int64_t r = e >> qshift;
-        if ((bits[r >> 5] >> (r & 31)) & 1u) continue;
+        dst[u] = A4 + e;
+        wp[u] = bits + (r >> 5);
+        sh[u] = int(r & 31);
       }
-      A4[e] = make_float4(0.f, 0.f, 0.f, 0.f);
+      uint32_t w[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) w[u] = (dst[u] && !full_fill) ? *wp[u] : 0u;
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (dst[u] && !((w[u] >> sh[u]) & 1u)) *dst[u] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
   }
   for (int64_t i = b_lo + tid; i < b_hi; i += nthr) {
@@ -920,11 +938,16 @@ __global__ __launch_bounds__(256) void batch_prep_kernel(const BatchTaskDev* __r
     }
     if (!full_fill) {
       zero_row(T.A, T.B.crd0[key], R);
-      int64_t q = bi * int64_t(NB) - 1;
-      if (q > T.B.nnz - 1) q = T.B.nnz - 1;
-      if (bi > 0 && q != p) {
-        const int32_t x1 = root_of_position(T.B, int32_t(q));
-        if (x1 != key) zero_row(T.A, T.B.crd0[x1], R);
+      // position p - 1 (the previous item's last nonzero) lies in another
+      // slice only when slice `key` starts exactly at p; its first nonzero
+      // is found by descending the pos arrays
+      if (bi > 0 && p > 0 && p == bi * int64_t(NB)) {
+        int32_t first = key;
+        for (int m = 1; m < N; ++m) first = T.B.pos[m][first];
+        if (first == p) {
+          const int32_t x1 = root_of_position(T.B, int32_t(p - 1));
+          if (x1 != key) zero_row(T.A, T.B.crd0[x1], R);
+        }
       }
     }
   }
