@@ -147,8 +147,11 @@ def profile_step(args):
     A = tt.DeviceArray.empty((B.dims[0], R))
     v = tt.variants().index(args.variant)
     H.reserve(R, v)
-    for _ in range(n):
-        H.mttkrp(fd, A, R, v)
+    # warm-up, then the steps back to back from the library's loop, as in the
+    # timed region of the main run
+    H.mttkrp_repeat(fd, A, R, args.warmup, v)
+    tt.synchronize()
+    H.mttkrp_repeat(fd, A, R, args.steps, v)
     tt.synchronize()
 
 
