@@ -1,8 +1,10 @@
-"""Parity at the benchmark workloads' full sizes: the 4096^3 / 500k-coordinate
-MTTKRP-3 (R = 32) with the variant the bench times and with the fixed default,
-the 100k^3 / 20M-nonzero TTV, and the 5-mode 50k^5 / 30M-nonzero MTTKRP
-(R = 32), each against the CPU oracle on the same seeded inputs (the bench
-itself only reports agreement between GPU paths)."""
+"""Parity at the benchmark workloads' full sizes, each against the CPU oracle
+on the same seeded inputs (the bench itself only reports agreement between
+GPU paths): the 4096^3 / 500k-coordinate MTTKRP-3 (R = 32) with the variant
+the bench times and with the fixed default; the 100k^3 / 20M-nonzero TTV;
+the 3/4/5-mode Zipf tensors of 100M / 60M / 30M nonzeros at R = 32; the
+non-root modes of the 20M tensor on one CSF; and every mode of the 64-tensor
+uneven batch through a replayed hipGraph."""
 
 import numpy as np
 import pytest
@@ -59,3 +61,70 @@ def test_mttkrp5_50k_30m_zipf_r32_full_size(gpu):
         H.mttkrp(fd, A, R, v)
         assert oracle.rel_l2(A.numpy(), ref) <= REL_L2_TOL, names[v]
     H.close()
+
+
+@pytest.mark.parametrize("order,dim,nnz", [(3, 500_000, 100_000_000), (4, 200_000, 60_000_000)])
+def test_zipf_tensor_full_size_r32(gpu, order, dim, nnz):
+    B = metrics.workload_zipf(order, dim, nnz, seed=1002 + order)
+    R = 32
+    facs = metrics.factors(B, R, 1035)
+    ref = oracle.mttkrp(B, facs, "taco", fp64=True)
+    H = tt.DeviceCsf(B)
+    fd = [tt.DeviceArray.from_numpy(f) for f in facs]
+    A = tt.DeviceArray.empty((B.dims[0], R))
+    names = tt.variants()
+    for v in ("ep_b512_g64", "ep_b256_g64"):
+        A.upload(np.full((B.dims[0], R), 3.0, np.float32))
+        H.mttkrp(fd, A, R, names.index(v))
+        assert oracle.rel_l2(A.numpy(), ref) <= REL_L2_TOL, (order, v)
+    H.close()
+
+
+def test_level_modes_20m_full_size(gpu):
+    """Levels 1 and 2 of the 20M-nonzero tensor on its one CSF against the
+    oracle on CSFs rooted at those modes."""
+    B = metrics.workload_zipf(3, 100_000, 20_000_000, seed=1001)
+    R = 32
+    rng = np.random.default_rng(1012)
+    F = [rng.uniform(-1, 1, (d, R)).astype(np.float32) for d in B.dims]
+    Fd = [tt.DeviceArray.from_numpy(f) for f in F]
+    H = tt.DeviceCsf(B)
+    for L in (1, 2):
+        A = tt.DeviceArray.from_numpy(np.full((B.dims[L], R), 3.0, np.float32))
+        H.mttkrp_level(L, [None if lv == L else Fd[lv] for lv in range(B.order)], A, R)
+        order_l = B.mode_levels(L)
+        ref = oracle.mttkrp(B.reorder(order_l), [F[k] for k in order_l[1:]], "taco", fp64=True)
+        assert oracle.rel_l2(A.numpy(), ref) <= REL_L2_TOL, L
+    H.close()
+
+
+def test_batch64_uneven_full_size_r32(gpu):
+    """Every mode of the 64-tensor uneven batch (259 tasks) in one replayed
+    hipGraph, outputs poisoned first, against the oracle task by task."""
+    tensors = metrics.workload_batch()
+    R = 32
+    rng = np.random.default_rng(1004)
+    tasks, checks, facs_d, outs, rows, keep = [], [], [], [], [], []
+    for B in tensors:
+        F = [rng.uniform(-1, 1, (d, R)).astype(np.float32) for d in B.dims]
+        Fd = [tt.DeviceArray.from_numpy(f) for f in F]
+        keep.append(Fd)
+        for m, Bm in enumerate(B.mode_csfs()):
+            rest = [k for k in range(B.order) if k != m]
+            tasks.append(tt.DeviceCsf(Bm))
+            checks.append((Bm, [F[k] for k in rest]))
+            facs_d.append([Fd[k] for k in rest])
+            outs.append(tt.DeviceArray.from_numpy(np.full((B.dims[m], R), 9.0, np.float32)))
+            rows.append(B.dims[m])
+    assert len(tasks) == 259
+    bt = tt.Batch(tasks, R)
+    bt.set_operands(facs_d, outs, rows)
+    bt.configure(1, 128, 64)
+    bt.capture()
+    bt.launch()
+    tt.synchronize()
+    for t, ((Bm, f), o) in enumerate(zip(checks, outs)):
+        assert oracle.rel_l2(o.numpy(), oracle.mttkrp(Bm, f, "taco", fp64=True)) <= REL_L2_TOL, t
+    bt.close()
+    for H in tasks:
+        H.close()
