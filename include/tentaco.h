@@ -399,7 +399,10 @@ int tentaco_batch_create(tentaco_csf* const* tasks, int ntasks, int R,
                          tentaco_batch** out);
 /* factors_dev: ntasks x 4 device pointers, task-major; slot m-1 of task t is
  * the factor of level m (slots past order-1 ignored). outs_dev[t] holds
- * out_rows[t] x R floats and is fully overwritten by each run. */
+ * out_rows[t] x R floats and is fully overwritten by each run. Builds each
+ * task's row bitmap (one bit per output row: rows that root nodes with
+ * nonzeros store, which the edge-parallel prep does not zero) on the null
+ * stream and synchronizes the device before returning. */
 int tentaco_batch_set_operands(tentaco_batch* b, const float* const* factors_dev,
                                float* const* outs_dev, const int32_t* out_rows);
 /* Tile configuration: family 0 edge-serial / 1 edge-parallel, block in
