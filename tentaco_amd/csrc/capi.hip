@@ -161,6 +161,7 @@ void free_handle(tentaco_csf* h) {
   (void)hipFree(h->stage_a);
   (void)hipFree(h->d_crd1_of_nnz);
   (void)hipFree(h->d_nnz_of_root);
+  (void)hipFree(h->d_pad);
   free_ttv_plan(&h->ttv);
   delete h;
 }
@@ -189,6 +190,25 @@ int ensure_expand3(tentaco_csf* h, hipStream_t st) {
   }
   h->d_crd1_of_nnz = kq;
   h->d_nnz_of_root = nzr;
+  // the padded slice layout: short fibres, packable coordinates, slices of at
+  // most 512 nonzeros and at most twice the nonzeros in slots; optional
+  const int64_t width = std::max<int64_t>(64, (h->stats.max_slice_nnz + 63) / 64 * 64);
+  const bool short_fibres = nnz * 4 <= int64_t(h->nodes[1]) * 5;
+  if (short_fibres && h->max_crd[1] < 65535 && h->max_crd[2] < 65535 && width <= 512 &&
+      int64_t(h->nodes[0]) * width <= 2 * nnz) {
+    uint2* pad = nullptr;
+    if (hipMalloc(reinterpret_cast<void**>(&pad), size_t(h->nodes[0]) * size_t(width) * sizeof(uint2)) ==
+        hipSuccess) {
+      if (launch_pad3(h->view(), nzr, int32_t(width), pad, st) == TENTACO_OK) {
+        h->d_pad = pad;
+        h->pad_width = int32_t(width);
+      } else {
+        (void)hipStreamSynchronize(st);
+        (void)hipFree(pad);
+        clear_error();
+      }
+    }
+  }
   return TENTACO_OK;
 }
 

@@ -27,6 +27,8 @@ struct tentaco_csf {
   // 3-mode expansion for the short-fibre slice path (built on first use)
   int32_t* d_crd1_of_nnz = nullptr;
   int32_t* d_nnz_of_root = nullptr;
+  uint2* d_pad = nullptr;  // padded slice layout (short-fibre 3-mode tensors)
+  int32_t pad_width = 0;
   // fibre-start bitmap and unit table of the TTV unit kernel (built on the
   // first TTV call outside stream capture)
   tt::TtvPlan ttv;
@@ -47,6 +49,8 @@ struct tentaco_csf {
     v.vals = d_vals;
     v.crd1_of_nnz = d_crd1_of_nnz;
     v.nnz_of_root = d_nnz_of_root;
+    v.pad = d_pad;
+    v.pad_width = pad_width;
     return v;
   }
 };

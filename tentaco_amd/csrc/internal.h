@@ -29,7 +29,14 @@ struct CsfDev {
   // every nonzero and the first nonzero of every root node (n0 + 1 entries)
   const int32_t* crd1_of_nnz = nullptr;
   const int32_t* nnz_of_root = nullptr;
+  // optional, 3-mode short fibres (built once per handle): every root node's
+  // nonzeros at the fixed slots [x * pad_width, x * pad_width + count) as
+  // ((level-1 coordinate << 16) | leaf coordinate, value bits); unused slots
+  // hold kPadEmpty
+  const uint2* pad = nullptr;
+  int32_t pad_width = 0;
 };
+constexpr uint32_t kPadEmpty = 0xFFFFFFFFu;
 
 // Factor matrices by level: M[m] for m = 1..order-1, row-major rows x R.
 struct FactorsDev {
@@ -74,6 +81,11 @@ int launch_mttkrp(const CsfDev& B, const FactorsDev& F, int R, float* A,
 // nnz_of_root: n0 + 1 entries) from the CSF's own pos/crd on `stream`.
 int launch_expand3(const CsfDev& B, int32_t* crd1_of_nnz, int32_t* nnz_of_root,
                    hipStream_t stream);
+// Fill the padded slice layout (n0 x width slots) of a 3-mode CSF whose
+// level-1 and leaf coordinates are below 65535 and whose root nodes hold at
+// most `width` nonzeros; nnz_of_root from launch_expand3 (same stream).
+int launch_pad3(const CsfDev& B, const int32_t* nnz_of_root, int32_t width, uint2* pad,
+                hipStream_t stream);
 
 // Enqueue the MTTKRP of the mode held at CSF level `level` (1..order-1) on
 // the same CSF: F.M[l] is the factor of level l for every l != level

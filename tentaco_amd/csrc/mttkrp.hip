@@ -671,6 +671,25 @@ __global__ __launch_bounds__(256) void expand3_kernel(const int32_t* __restrict_
   for (int64_t x = tid; x <= n0; x += nthr) nzr[x] = pos2[pos1[x]];
 }
 
+__global__ __launch_bounds__(256) void pad3_fill_kernel(uint2* __restrict__ pad, int64_t n) {
+  const int64_t nthr = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += nthr)
+    pad[i] = make_uint2(kPadEmpty, 0u);
+}
+
+__global__ __launch_bounds__(256) void pad3_build_kernel(CsfDev B, const int32_t* __restrict__ nzr,
+                                                         int32_t width, uint2* __restrict__ pad) {
+  const int64_t nthr = int64_t(gridDim.x) * blockDim.x;
+  const int32_t nf = B.nodes[1];
+  for (int64_t f = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; f < nf; f += nthr) {
+    const int32_t x = search_le(B.pos[1], 0, B.nodes[0] - 1, int32_t(f));
+    const uint32_t hi = uint32_t(B.crd[1][f]) << 16;
+    uint2* slot = pad + int64_t(x) * width - nzr[x];
+    for (int32_t q = B.pos[2][f]; q < B.pos[2][f + 1]; ++q)
+      slot[q] = make_uint2(hi | uint32_t(B.crd[2][q]), __float_as_uint(B.vals[q]));
+  }
+}
+
 // One workgroup per root node; its blockDim/64 wavefronts split the slice
 // (64-child chunks round-robin), reduce their rows with xor shuffles, then
 // across wavefronts through LDS; one store per row, no atomics.
@@ -803,6 +822,82 @@ __global__ __launch_bounds__(512) void slice_short_kernel(CsfDev B, FactorsDev F
       }
     }
     RO::store(A, B.crd0[x], c, Rr, acc);
+  }
+}
+
+// Slice-row kernel over the padded layout: one workgroup of pad_width
+// threads per root node, each thread holding one slot, so the slice's
+// nonzeros are read at addresses fixed at launch (no offset load first).
+// Row groups take the wave's slots by shuffles as in slice_short_kernel,
+// empty slots issue no gathers, and the same wave and LDS reduction ends in
+// one store per row.
+template <int R>
+__global__ __launch_bounds__(512) void slice_pad_kernel(CsfDev B, FactorsDev F,
+                                                        float* __restrict__ A, int Rr) {
+  using RO = Row<R>;
+  using V = typename RO::V;
+  constexpr int NV = RO::NV;
+  constexpr int LPR = RO::LPR;
+  constexpr int NG = 64 / LPR;
+  constexpr int NT = 64 / NG;
+  __shared__ __align__(16) float red[8][Lay<R>::kMaxRank];
+  const int32_t x = blockIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int nwv = blockDim.x >> 6;
+  const int gi = lane / LPR;
+  const int c = lane % LPR;
+  const int32_t row = B.crd0[x];
+  const uint2 e = B.pad[int64_t(x) * blockDim.x + threadIdx.x];
+  const float* __restrict__ Cm = F.M[1];
+  const float* __restrict__ Dm = F.M[2];
+  V cr[NT][NV], dr[NT][NV];
+  float vt[NT];
+#pragma unroll
+  for (int u = 0; u < NT; ++u) {
+    const int src = u * NG + gi;
+    const uint32_t kl = __shfl(e.x, src);
+    vt[u] = __uint_as_float(__shfl(e.y, src));
+    if (kl != kPadEmpty) {
+      RO::load(cr[u], Cm, int32_t(kl >> 16), c, Rr);
+      RO::load(dr[u], Dm, int32_t(kl & 0xFFFFu), c, Rr);
+    } else {
+      RO::zero(cr[u]);
+      RO::zero(dr[u]);
+      vt[u] = 0.f;
+    }
+  }
+  V acc[NV];
+  RO::zero(acc);
+#pragma unroll
+  for (int u = 0; u < NT; ++u) {
+    V w[NV];
+    RO::zero(w);
+    RO::axpy(w, vt[u], dr[u]);
+    RO::hfma(acc, w, cr[u]);
+  }
+  RO::wave_sum(acc);
+  if (nwv == 1) {
+    if (gi == 0) RO::store(A, row, c, Rr, acc);
+    return;
+  }
+  if (gi == 0) {
+#pragma unroll
+    for (int j = 0; j < NV; ++j) *reinterpret_cast<float4*>(&red[wv][RO::col(j, c)]) = acc[j];
+  }
+  __syncthreads();
+  if (wv == 0 && gi == 0) {
+    for (int w = 1; w < nwv; ++w) {
+#pragma unroll
+      for (int j = 0; j < NV; ++j) {
+        const float4 o = *reinterpret_cast<const float4*>(&red[w][RO::col(j, c)]);
+        acc[j].x += o.x;
+        acc[j].y += o.y;
+        acc[j].z += o.z;
+        acc[j].w += o.w;
+      }
+    }
+    RO::store(A, row, c, Rr, acc);
   }
 }
 
@@ -1131,6 +1226,12 @@ int launch_typed(const CsfDev& B, const FactorsDev& F, int Rr, float* A,
         // short fibres (nnz <= 1.25 x fibres) with the expansion built: the
         // kernel that skips the pos -> crd chain
         if (B.crd1_of_nnz && int64_t(B.nnz) * 4 <= int64_t(B.nodes[1]) * 5) {
+          if (B.pad && B.pad_width <= vr.block) {
+            slice_pad_kernel<R><<<B.nodes[0], B.pad_width, 0, st>>>(B, F, A, Rr);
+            TT_HIP_CHECK(hipGetLastError());
+            ktimer_end(st, "slice_pad_kernel");
+            return TENTACO_OK;
+          }
           slice_short_kernel<R><<<B.nodes[0], vr.block, 0, st>>>(B, F, A, Rr);
           TT_HIP_CHECK(hipGetLastError());
           ktimer_end(st, "slice_short_kernel");
@@ -1247,6 +1348,18 @@ int launch_expand3(const CsfDev& B, int32_t* kq, int32_t* nzr, hipStream_t st) {
   expand3_kernel<<<grid_for(work, 256, 16384), 256, 0, st>>>(B.pos[1], B.pos[2], B.crd[1],
                                                              B.nodes[0], B.nodes[1], kq, nzr);
   TT_HIP_CHECK(hipGetLastError());
+  return TENTACO_OK;
+}
+
+int launch_pad3(const CsfDev& B, const int32_t* nzr, int32_t width, uint2* pad, hipStream_t st) {
+  if (B.order != 3) return fail(TENTACO_EINVAL, "padded layout is for 3-mode tensors");
+  const int64_t n = int64_t(B.nodes[0]) * width;
+  pad3_fill_kernel<<<grid_for(n, 256, 16384), 256, 0, st>>>(pad, n);
+  TT_HIP_CHECK(hipGetLastError());
+  if (B.nodes[1] > 0) {
+    pad3_build_kernel<<<grid_for(B.nodes[1], 256, 16384), 256, 0, st>>>(B, nzr, width, pad);
+    TT_HIP_CHECK(hipGetLastError());
+  }
   return TENTACO_OK;
 }
 

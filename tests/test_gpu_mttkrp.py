@@ -281,3 +281,64 @@ def test_slice_row_short_fibres(gpu, R, compressed_root):
         e = oracle.rel_l2(A.numpy(), ref)
         assert e <= REL_L2_TOL, (vname, R, compressed_root, e)
     H.close()
+
+
+def _slices_coo(rng, sizes, dims):
+    """COO whose root slice i holds sizes[i] nonzeros (level-1 and leaf
+    coordinates uniform, so nearly every fibre holds one nonzero)."""
+    sizes = np.asarray(sizes)
+    n = int(sizes.sum())
+    c0 = np.repeat(np.arange(sizes.size), sizes)
+    coords = np.stack([c0, rng.integers(0, dims[1], n), rng.integers(0, dims[2], n)]).astype(np.int32)
+    return tt.Csf.from_coo(dims, coords, rng.uniform(-1, 1, n).astype(np.float32))
+
+
+@pytest.mark.parametrize("R", [16, 32, 64, 128])
+def test_slice_row_padded_layout(gpu, R):
+    """Short-fibre 3-mode tensors whose slices fit the padded layout (width =
+    longest slice rounded up to 64, at most twice the nonzeros in slots) run
+    slice_pad_kernel for every slice-row block at least as wide: empty slices
+    at the start, in runs and at the end, slices up to the width, one
+    nonzero. Against the oracle with poisoned outputs, exact zeros on empty
+    rows and bitwise-equal repeats; a tensor whose padding would exceed
+    twice its nonzeros takes slice_short_kernel."""
+    rng = np.random.default_rng(600 + R)
+    dims = (0, 30000, 20000)
+    cases = []
+    sizes = rng.integers(60, 120, 3000)
+    sizes[:5] = 0
+    sizes[-7:] = 0
+    sizes[rng.random(3000) < 0.1] = 0
+    cases.append(sizes)
+    sizes = rng.integers(300, 513, 400)
+    sizes[rng.random(400) < 0.05] = 0
+    cases.append(sizes)
+    cases.append(np.array([0, 0, 1, 0]))
+    sizes = np.where(rng.random(2500) < 0.5, 1, rng.integers(100, 257, 2500))
+    cases.append(sizes)  # 2500 x 256 slots > 2 x nnz: no padded layout
+    names = tt.variants()
+    for ci, sizes in enumerate(cases):
+        d = (sizes.size, dims[1], dims[2])
+        B = _slices_coo(rng, sizes, d)
+        assert B.nnz * 4 <= B.nodes(1) * 5
+        width = max(64, -(-int(sizes.max()) // 64) * 64)
+        padded = width <= 512 and sizes.size * width <= 2 * B.nnz
+        facs = [rng.uniform(-1, 1, (x, R)).astype(np.float32) for x in d[1:]]
+        ref = oracle.mttkrp(B, facs, "taco", fp64=True)
+        empty = np.flatnonzero(sizes == 0)
+        H = tt.DeviceCsf(B)
+        fd = [tt.DeviceArray.from_numpy(f) for f in facs]
+        for blk in (128, 256, 512):
+            A = tt.DeviceArray.from_numpy(np.full((d[0], R), 3.0, np.float32))
+            tt.ktimer_start()
+            H.mttkrp(fd, A, R, names.index(f"sr_b{blk}"))
+            _, _, kname = tt.ktimer_stop()
+            want = "slice_pad_kernel" if padded and width <= blk else "slice_short_kernel"
+            assert kname == want, (ci, blk, kname)
+            got = A.numpy()
+            assert oracle.rel_l2(got, ref) <= REL_L2_TOL, (ci, blk, R)
+            assert np.all(got[empty] == 0.0), (ci, blk, R)
+            A.upload(np.full((d[0], R), -1.0, np.float32))
+            H.mttkrp(fd, A, R, names.index(f"sr_b{blk}"))
+            assert np.array_equal(A.numpy(), got), (ci, blk, R)
+        H.close()
