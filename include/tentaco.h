@@ -207,7 +207,10 @@ int tentaco_csf_free(tentaco_csf* h);
 int tentaco_csf_info(const tentaco_csf* h, int* order, int64_t* nnz,
                      int32_t* nodes);
 /* Reserve the per-handle workspace a variant needs, so that later calls on
- * this handle allocate nothing (required before graph capture). */
+ * this handle allocate nothing (required before graph capture). For 3-mode
+ * handles it also builds the slice-row structures: the expansion (4 B per
+ * nonzero plus 4 B per root node) and, for short fibres whose slices fit,
+ * the padded slice layout (8 B per slot, at most 2 slots per nonzero). */
 int tentaco_csf_reserve(tentaco_csf* h, int R, int variant);
 
 /* MTTKRP on device buffers: factors_dev[m-1] (rows_m x R, row-major) for
@@ -474,6 +477,7 @@ int tentaco_timer_stop(void* stream, float* ms);
 
 /* Kernel timer: while on, every call records a hipEvent pair on its launch
  * stream around its main kernel only (slice_kernel, slice_short_kernel,
+ * slice_pad_kernel,
  * edge_kernel, level_kernel, ttv_kernel, batch_kernel or copy_kernel; not the
  * partition/zero pre-pass), at most 4096
  * launches; never inside a stream capture. stop() turns it off and returns
