@@ -284,7 +284,7 @@ int tentaco_cpd_free(tentaco_cpd* c);
 /* No reference counterpart: it replaces running the exhaustive sweep
  * (codegen.py:44-51 + ebsr_32_tune.sh / ebpr_*_tune.sh) for every tensor with
  * a prediction from the tensor's structure, trained on the sweep's own
- * timings; the sweep stays as the fallback and as the judge of its picks. */
+ * timings; the sweep stays as the fallback and as the check on its picks. */
 #define TENTACO_NUM_FEATURES 24
 #define TENTACO_MLP_HIDDEN 64
 /* The 24 features of an uploaded CSF at rank R:
