@@ -620,6 +620,7 @@ def run_batch_workload(tt, metrics, K):
     bt.set_operands(facs_d, outs, rows)
     (fam, blk, grp), tune_ms = bt.autotune(reps=3)
     sched = bt.schedule
+    order_tiles = {N: bt.config(N) for N in sorted({B.order for B in tensors})}
     items, grid, nb = bt.items()
     bt.run()
     tt.synchronize()
@@ -654,7 +655,10 @@ def run_batch_workload(tt, metrics, K):
         "batched_gbps": round(balg / (g_ms * 1e-3) / 1e9, 2),
         "config": {"family": ["edge-serial", "edge-parallel"][fam], "block": blk, "group": grp,
                    "schedule": ["work counter", "static cyclic"][sched],
-                   "items": items, "persistent_grid": grid, "nnz_per_item": nb},
+                   "items": items, "persistent_grid": grid, "nnz_per_item": nb,
+                   "tile_per_order": {str(N): dict(zip(("family", "block", "group", "schedule"),
+                                                        order_tiles[N])) for N in order_tiles},
+                   "autotune_final_ms": round(tune_ms, 4)},
         "max_rel_l2_batched_vs_per_task": max(eager_err, graph_err),
     }
     log(f"batch: per-task {per_ms:.3f} ms, batched graph {g_ms:.3f} ms "

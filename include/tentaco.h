@@ -408,29 +408,43 @@ int tentaco_batch_create(tentaco_csf* const* tasks, int ntasks, int R,
  * stream and synchronizes the device before returning. */
 int tentaco_batch_set_operands(tentaco_batch* b, const float* const* factors_dev,
                                float* const* outs_dev, const int32_t* out_rows);
-/* Tile configuration: family 0 edge-serial / 1 edge-parallel, block in
- * {128, 256, 512}, group = nonzeros per lane group. Drops a captured graph. */
+/* Tile configuration of every order group: family 0 edge-serial / 1
+ * edge-parallel, block in {128, 256, 512}, group = nonzeros per lane group.
+ * Drops a captured graph. */
 int tentaco_batch_configure(tentaco_batch* b, int family, int block, int group);
+/* The tile and schedule (see tentaco_batch_set_schedule) of the tasks of one
+ * tensor order (2..5) only; each order group runs its own persistent launch,
+ * so the groups may differ. Drops a captured graph. */
+int tentaco_batch_configure_order(tentaco_batch* b, int order, int family, int block,
+                                  int group, int schedule);
+/* The tile and schedule an order group runs with. */
+int tentaco_batch_get_config(const tentaco_batch* b, int order, int* family, int* block,
+                             int* group, int* schedule);
 /* Enqueue prep + persistent kernel on `stream`. */
 int tentaco_batch_run(tentaco_batch* b, void* stream);
 /* Capture tentaco_batch_run into an instantiated hipGraph (kernels only). */
 int tentaco_batch_capture(tentaco_batch* b, void* stream);
 /* Launch the captured graph on `stream`. */
 int tentaco_batch_launch(tentaco_batch* b, void* stream);
-/* Work items, persistent grid size and nonzeros per item of the current
- * configuration. */
+/* Work items, the largest persistent grid and the nonzeros per item of the
+ * order group with the most items, in the current configuration. */
 int tentaco_batch_items(const tentaco_batch* b, int32_t* items, int32_t* grid,
                         int32_t* nb);
-/* Time every tile configuration (2 schedules x 2 families x 3 blocks x 5
- * groups), keep the fastest configured and return it. Not for graph capture. */
+/* Time every tile configuration for all orders at once (2 schedules x 2
+ * families x 3 blocks x 5 groups) and keep the fastest, returned in
+ * best_family / best_block / best_group. Then time every tile and schedule of
+ * each order group in turn with the other groups held, keeping what is
+ * faster (tentaco_batch_get_config reads the result); best_ms is the final
+ * time. Not for graph capture. */
 int tentaco_batch_autotune(tentaco_batch* b, int reps, void* stream,
                            int* best_family, int* best_block, int* best_group,
                            float* best_ms);
 int tentaco_batch_free(tentaco_batch* b);
-/* How persistent workgroups take items: 0 = from a device work counter in
- * arrival order (the default), 1 = static cyclic assignment (item = block +
- * k * grid, no shared counter). Kept across configure(); autotune searches
- * both and leaves the faster set. */
+/* How persistent workgroups take items, for every order group: 0 = from a
+ * device work counter in arrival order (the default), 1 = static cyclic
+ * assignment (item = block + k * grid, no shared counter). Kept across
+ * configure(); autotune searches both, jointly and per order. get_schedule
+ * reports the order group with the most items. */
 int tentaco_batch_set_schedule(tentaco_batch* b, int schedule);
 int tentaco_batch_get_schedule(const tentaco_batch* b, int* schedule);
 

@@ -147,6 +147,8 @@ def lib() -> ctypes.CDLL:
         "tentaco_batch_create": (c_int, [POINTER(c_void_p), c_int, c_int, POINTER(c_void_p)]),
         "tentaco_batch_set_operands": (c_int, [c_void_p, POINTER(c_void_p), POINTER(c_void_p), POINTER(c_int32)]),
         "tentaco_batch_configure": (c_int, [c_void_p, c_int, c_int, c_int]),
+        "tentaco_batch_configure_order": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int]),
+        "tentaco_batch_get_config": (c_int, [c_void_p, c_int, POINTER(c_int), POINTER(c_int), POINTER(c_int), POINTER(c_int)]),
         "tentaco_batch_run": (c_int, [c_void_p, c_void_p]),
         "tentaco_batch_capture": (c_int, [c_void_p, c_void_p]),
         "tentaco_batch_launch": (c_int, [c_void_p, c_void_p]),
@@ -639,6 +641,19 @@ class Batch:
     def configure(self, family: int, block: int, group: int):
         _check(lib().tentaco_batch_configure(self.h, family, block, group))
 
+    def configure_order(self, order: int, family: int, block: int, group: int,
+                        schedule: int = 0):
+        """The tile and schedule of one tensor order's tasks only
+        (tentaco_batch_configure_order)."""
+        _check(lib().tentaco_batch_configure_order(self.h, order, family, block, group, schedule))
+
+    def config(self, order: int):
+        """(family, block, group, schedule) that an order group runs with."""
+        f, b, g, sc = c_int(), c_int(), c_int(), c_int()
+        _check(lib().tentaco_batch_get_config(self.h, order, ctypes.byref(f), ctypes.byref(b),
+                                              ctypes.byref(g), ctypes.byref(sc)))
+        return f.value, b.value, g.value, sc.value
+
     def run(self, stream=None):
         _check(lib().tentaco_batch_run(self.h, stream))
 
@@ -664,6 +679,9 @@ class Batch:
         return s.value
 
     def autotune(self, reps: int = 3, stream=None):
+        """The joint sweep's best tile for all orders, then each order group's
+        own tile (read with config(order)); returns (joint best tile,
+        final ms)."""
         f, b, g, ms = c_int(), c_int(), c_int(), c_float()
         _check(lib().tentaco_batch_autotune(self.h, reps, stream, ctypes.byref(f), ctypes.byref(b),
                                             ctypes.byref(g), ctypes.byref(ms)))

@@ -66,22 +66,29 @@ void free_tables(tentaco_batch* b) {
   b->brk_bytes = 0;
 }
 
-// Rebuild the item/boundary/fill tables and the bracket workspace for a tile
-// configuration, and upload the task table.
-int configure(tentaco_batch* b, int family, int block, int group) {
+int check_tile(int family, int block, int group) {
   if (family != kEdgeSerial && family != kEdgeParallel)
     return fail(TENTACO_EINVAL, "batch family must be 0 (edge-serial) or 1 (edge-parallel)");
   if (block != 128 && block != 256 && block != 512)
     return fail(TENTACO_EINVAL, "batch block %d not in {128,256,512}", block);
   if (group < 1 || group > 1024) return fail(TENTACO_EINVAL, "batch group %d", group);
+  return TENTACO_OK;
+}
+
+// Rebuild the item/boundary/fill tables and the bracket workspace for the
+// per-order tiles in b->L (a task's items use its order's nonzeros per item),
+// and upload the task table.
+int configure_tables(tentaco_batch* b) {
   if (!b->have_operands) return fail(TENTACO_EINVAL, "set operands before configuring");
   drop_graph(b);
+  BatchLaunch& L = b->L;
   const int lpr = lanes_per_row(b->R);
-  const int64_t NB = int64_t(block / lpr) * group;
+  for (int N = 0; N <= kMaxOrder; ++N) L.nb[N] = int32_t((L.blk[N] / lpr) * L.grp[N]);
   std::vector<int64_t> bnd(b->ntasks + 1, 0), item(b->ntasks + 1, 0), brk_off(b->ntasks + 1, 0);
   for (int t = 0; t < b->ntasks; ++t) {
     const tentaco_csf* h = b->tasks[t];
     const int64_t nnz = h->nodes[h->order - 1];
+    const int64_t NB = L.nb[h->order];
     const int64_t nblk = (nnz + NB - 1) / NB;
     item[t + 1] = item[t] + nblk;
     bnd[t + 1] = bnd[t] + nblk + 1;
@@ -103,7 +110,6 @@ int configure(tentaco_batch* b, int family, int block, int group) {
   TT_HIP_CHECK(hipMemcpy(b->d_item_off, item.data(), sizeof(int64_t) * (b->ntasks + 1),
                          hipMemcpyHostToDevice));
   b->h_bnd = bnd;
-  BatchLaunch& L = b->L;
   L.h_bnd_off = b->h_bnd.data();
   L.h_fill_off = b->h_fill.data();
   L.tasks = b->d_tasks;
@@ -113,12 +119,7 @@ int configure(tentaco_batch* b, int family, int block, int group) {
   L.item_off = b->d_item_off;
   L.total_items = int32_t(item[b->ntasks]);
   L.counter = b->d_counter;
-  L.full_fill = family != kEdgeParallel;
-  L.family = family;
-  L.block = block;
-  L.group = group;
   L.R = b->R;
-  L.NB = int32_t(NB);
   int dev = 0;
   TT_HIP_CHECK(hipGetDevice(&dev));
   hipDeviceProp_t prop;
@@ -135,12 +136,32 @@ int configure(tentaco_batch* b, int family, int block, int group) {
     L.grp_grid[N] = 0;
     if (L.grp_item[N + 1] <= L.grp_item[N]) continue;
     int per_cu = 1;
-    rc = batch_occupancy(family, block, b->R, int32_t(NB), N, &per_cu);
+    rc = batch_occupancy(L.fam[N], L.blk[N], b->R, L.nb[N], N, &per_cu);
     if (rc) return rc;
     L.grp_grid[N] = std::max(1, per_cu) * prop.multiProcessorCount;
     L.grid = std::max(L.grid, L.grp_grid[N]);
   }
   return TENTACO_OK;
+}
+
+void set_tile(tentaco_batch* b, int order, int family, int block, int group) {
+  b->L.fam[order] = family;
+  b->L.blk[order] = block;
+  b->L.grp[order] = group;
+}
+
+// The order group with the most items (the one whose settings the
+// single-valued queries report).
+int main_order(const BatchLaunch& L) {
+  int best = kMaxOrder, most = -1;
+  for (int N = 2; N <= kMaxOrder; ++N) {
+    const int32_t n = L.grp_item[N + 1] - L.grp_item[N];
+    if (n > most) {
+      most = n;
+      best = N;
+    }
+  }
+  return best;
 }
 
 }  // namespace
@@ -233,29 +254,62 @@ int tentaco_batch_set_operands(tentaco_batch* b, const float* const* factors_dev
   b->h_fill = fill;
   b->L.h_fill_off = b->h_fill.data();
   b->have_operands = true;
-  const bool configured = b->L.NB != 0;
-  return configure(b, configured ? b->L.family : kEdgeParallel, configured ? b->L.block : 256,
-                   configured ? b->L.group : 16);
+  if (b->L.blk[kMaxOrder] == 0)  // never configured: edge-parallel, 256 threads, group 16
+    for (int N = 0; N <= kMaxOrder; ++N) set_tile(b, N, kEdgeParallel, 256, 16);
+  return configure_tables(b);
 }
 
 int tentaco_batch_configure(tentaco_batch* b, int family, int block, int group) {
   clear_error();
   if (!b) return fail(TENTACO_EINVAL, "batch is NULL");
-  return configure(b, family, block, group);
+  int rc = check_tile(family, block, group);
+  if (rc) return rc;
+  if (!b->have_operands) return fail(TENTACO_EINVAL, "set operands before configuring");
+  for (int N = 0; N <= kMaxOrder; ++N) set_tile(b, N, family, block, group);
+  return configure_tables(b);
+}
+
+int tentaco_batch_configure_order(tentaco_batch* b, int order, int family, int block, int group,
+                                  int schedule) {
+  clear_error();
+  if (!b) return fail(TENTACO_EINVAL, "batch is NULL");
+  if (order < 2 || order > kMaxOrder)
+    return fail(TENTACO_EINVAL, "order %d outside 2..%d", order, kMaxOrder);
+  int rc = check_tile(family, block, group);
+  if (rc) return rc;
+  if (schedule != 0 && schedule != 1) return fail(TENTACO_EINVAL, "schedule must be 0 or 1");
+  if (!b->have_operands) return fail(TENTACO_EINVAL, "set operands before configuring");
+  set_tile(b, order, family, block, group);
+  b->L.sch[order] = schedule;
+  return configure_tables(b);
+}
+
+int tentaco_batch_get_config(const tentaco_batch* b, int order, int* family, int* block,
+                             int* group, int* schedule) {
+  if (!b) return fail(TENTACO_EINVAL, "batch is NULL");
+  if (order < 2 || order > kMaxOrder)
+    return fail(TENTACO_EINVAL, "order %d outside 2..%d", order, kMaxOrder);
+  if (family) *family = b->L.fam[order];
+  if (block) *block = b->L.blk[order];
+  if (group) *group = b->L.grp[order];
+  if (schedule) *schedule = b->L.sch[order];
+  return TENTACO_OK;
 }
 
 int tentaco_batch_set_schedule(tentaco_batch* b, int schedule) {
   clear_error();
   if (!b) return fail(TENTACO_EINVAL, "batch is NULL");
   if (schedule != 0 && schedule != 1) return fail(TENTACO_EINVAL, "schedule must be 0 or 1");
-  if (b->L.schedule != schedule) drop_graph(b);
-  b->L.schedule = schedule;
+  for (int N = 0; N <= kMaxOrder; ++N) {
+    if (b->L.sch[N] != schedule) drop_graph(b);
+    b->L.sch[N] = schedule;
+  }
   return TENTACO_OK;
 }
 
 int tentaco_batch_get_schedule(const tentaco_batch* b, int* schedule) {
   if (!b || !schedule) return fail(TENTACO_EINVAL, "NULL argument");
-  *schedule = b->L.schedule;
+  *schedule = b->L.sch[main_order(b->L)];
   return TENTACO_OK;
 }
 
@@ -299,7 +353,7 @@ int tentaco_batch_items(const tentaco_batch* b, int32_t* items, int32_t* grid, i
   if (!b) return fail(TENTACO_EINVAL, "batch is NULL");
   if (items) *items = b->L.total_items;
   if (grid) *grid = b->L.grid;
-  if (nb) *nb = b->L.NB;
+  if (nb) *nb = b->L.nb[main_order(b->L)];
   return TENTACO_OK;
 }
 
@@ -312,48 +366,87 @@ int tentaco_batch_autotune(tentaco_batch* b, int reps, void* stream, int* best_f
   hipEvent_t e0, e1;
   TT_HIP_CHECK(hipEventCreate(&e0));
   TT_HIP_CHECK(hipEventCreate(&e1));
+  struct Ev {
+    hipEvent_t a, b;
+    ~Ev() {
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
+    }
+  } ev{e0, e1};
+  std::vector<float> t(static_cast<size_t>(reps));
+  // median of `reps` timed runs after a warm-up of the current configuration
+  auto timed = [&](float* med) -> int {
+    int rc = tentaco_batch_run(b, stream);
+    for (int r = 0; r < reps && !rc; ++r) {
+      (void)hipEventRecord(e0, st);
+      rc = tentaco_batch_run(b, stream);
+      (void)hipEventRecord(e1, st);
+      (void)hipEventSynchronize(e1);
+      (void)hipEventElapsedTime(&t[size_t(r)], e0, e1);
+    }
+    if (rc) return rc;
+    std::sort(t.begin(), t.end());
+    *med = t[t.size() / 2];
+    return TENTACO_OK;
+  };
+  // 1. one tile for every order, with either schedule
   float bm = -1.f;
   int bf = -1, bb = 0, bg = 0, bs = 0;
-  std::vector<float> t(static_cast<size_t>(reps));
   for (int sched = 0; sched < 2; ++sched)
-  for (int fam = 0; fam < 2; ++fam)
-    for (int blk : {128, 256, 512})
-      for (int grp : {4, 8, 16, 32, 64}) {
-        int rc = tentaco_batch_configure(b, fam, blk, grp);
-        if (!rc) rc = tentaco_batch_set_schedule(b, sched);
-        if (!rc) rc = tentaco_batch_run(b, stream);  // warm-up
-        for (int r = 0; r < reps && !rc; ++r) {
-          (void)hipEventRecord(e0, st);
-          rc = tentaco_batch_run(b, stream);
-          (void)hipEventRecord(e1, st);
-          (void)hipEventSynchronize(e1);
-          (void)hipEventElapsedTime(&t[size_t(r)], e0, e1);
+    for (int fam = 0; fam < 2; ++fam)
+      for (int blk : {128, 256, 512})
+        for (int grp : {4, 8, 16, 32, 64}) {
+          float med = 0.f;
+          int rc = tentaco_batch_configure(b, fam, blk, grp);
+          if (!rc) rc = tentaco_batch_set_schedule(b, sched);
+          if (!rc) rc = timed(&med);
+          if (rc == TENTACO_EUNSUPPORTED) {
+            clear_error();
+            continue;
+          }
+          if (rc) return rc;
+          if (bf < 0 || med < bm) {
+            bm = med;
+            bf = fam;
+            bb = blk;
+            bg = grp;
+            bs = sched;
+          }
         }
-        if (rc == TENTACO_EUNSUPPORTED) {
-          clear_error();
-          continue;
-        }
-        if (rc) {
-          (void)hipEventDestroy(e0);
-          (void)hipEventDestroy(e1);
-          return rc;
-        }
-        std::sort(t.begin(), t.end());
-        const float med = t[t.size() / 2];
-        if (bf < 0 || med < bm) {
-          bm = med;
-          bf = fam;
-          bb = blk;
-          bg = grp;
-          bs = sched;
-        }
-      }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   if (bf < 0) return fail(TENTACO_EUNSUPPORTED, "no batch configuration ran");
   int rc = tentaco_batch_configure(b, bf, bb, bg);
   if (!rc) rc = tentaco_batch_set_schedule(b, bs);
   if (rc) return rc;
+  // 2. each order group's own tile, the others held at their best (the order
+  // launches run one after another, so their times add)
+  for (int N = 2; N <= kMaxOrder; ++N) {
+    if (b->L.grp_task[N + 1] <= b->L.grp_task[N]) continue;
+    int of = b->L.fam[N], ob = b->L.blk[N], og = b->L.grp[N], os = b->L.sch[N];
+    const int f0 = of, b0 = ob, g0 = og, s0 = os;
+    for (int sched = 0; sched < 2; ++sched)
+    for (int fam = 0; fam < 2; ++fam)
+      for (int blk : {128, 256, 512})
+        for (int grp : {4, 8, 16, 32, 64}) {
+          if (fam == f0 && blk == b0 && grp == g0 && sched == s0) continue;
+          float med = 0.f;
+          rc = tentaco_batch_configure_order(b, N, fam, blk, grp, sched);
+          if (!rc) rc = timed(&med);
+          if (rc == TENTACO_EUNSUPPORTED) {
+            clear_error();
+            continue;
+          }
+          if (rc) return rc;
+          if (med < bm) {
+            bm = med;
+            of = fam;
+            ob = blk;
+            og = grp;
+            os = sched;
+          }
+        }
+    rc = tentaco_batch_configure_order(b, N, of, ob, og, os);
+    if (rc) return rc;
+  }
   if (best_family) *best_family = bf;
   if (best_block) *best_block = bb;
   if (best_group) *best_group = bg;

@@ -159,15 +159,19 @@ struct BatchLaunch {
   const int64_t* item_off = nullptr;    // device, prefix of items per task (ntasks + 1)
   int32_t total_items = 0;
   int32_t* counter = nullptr;           // device work counter
-  // false for the edge-parallel family: the prep then skips the output rows
-  // that root nodes with nonzeros store (their bits in the tasks' rowbits),
-  // except the rows of the slices holding an item's first or last nonzero,
-  // which take atomics. true: whole outputs are zero-filled.
-  bool full_fill = true;
-  int family = 1, block = 256, group = 16, R = 32;
-  int schedule = 0;  // 0: device work counter, 1: static cyclic items
-  int32_t NB = 0;
-  int grid = 0;
+  // Tile per order (index = order): family, block, group and nonzeros per
+  // item (nb = block / lanes-per-row * group). For the edge-parallel family
+  // the prep skips the output rows that root nodes with nonzeros store (their
+  // bits in the tasks' rowbits), except the rows of the slices holding an
+  // item's first or last nonzero, which take atomics; the edge-serial family
+  // zero-fills whole outputs.
+  // sch[N] is how the order's persistent workgroups take items: 0 from a
+  // device work counter, 1 static cyclic.
+  int fam[kMaxOrder + 1] = {}, blk[kMaxOrder + 1] = {}, grp[kMaxOrder + 1] = {};
+  int sch[kMaxOrder + 1] = {};
+  int32_t nb[kMaxOrder + 1] = {};
+  int R = 32;
+  int grid = 0;  // the largest per-order grid
   // the device task table is sorted by order: tasks [grp_task[N],
   // grp_task[N+1]) of order N hold items [grp_item[N], grp_item[N+1]) and run
   // in their own persistent launch of grp_grid[N] workgroups; counter holds

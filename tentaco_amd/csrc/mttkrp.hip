@@ -1138,18 +1138,19 @@ int launch_batch_order(const BatchLaunch& L, hipStream_t st) {
   const int32_t i_lo = L.grp_item[N], i_hi = L.grp_item[N + 1];
   if (i_hi <= i_lo) return TENTACO_OK;
   const int lpr = Row<R>::LPR;
-  size_t lds = edge_stage_ints(N, L.NB) * sizeof(int32_t);
-  if (L.family == kEdgeParallel) lds += edge_ep_bytes(L.block, lpr, L.R);
+  const int fam = L.fam[N], blk = L.blk[N];
+  size_t lds = edge_stage_ints(N, L.nb[N]) * sizeof(int32_t);
+  if (fam == kEdgeParallel) lds += edge_ep_bytes(blk, lpr, L.R);
   if (lds > 65536) return fail(TENTACO_EUNSUPPORTED, "batch LDS %zu B", lds);
   const int grid = std::max(1, std::min<int>(L.grp_grid[N], i_hi - i_lo));
-  if (L.family == kEdgeParallel)
-    batch_kernel<N, R, true><<<grid, L.block, lds, st>>>(
+  if (fam == kEdgeParallel)
+    batch_kernel<N, R, true><<<grid, blk, lds, st>>>(
         L.tasks, L.grp_task[N], L.grp_task[N + 1], L.item_off, i_lo, i_hi, L.counter + N, L.R,
-        L.group, L.NB, L.schedule == 0);
+        L.grp[N], L.nb[N], L.sch[N] == 0);
   else
-    batch_kernel<N, R, false><<<grid, L.block, lds, st>>>(
+    batch_kernel<N, R, false><<<grid, blk, lds, st>>>(
         L.tasks, L.grp_task[N], L.grp_task[N + 1], L.item_off, i_lo, i_hi, L.counter + N, L.R,
-        L.group, L.NB, L.schedule == 0);
+        L.grp[N], L.nb[N], L.sch[N] == 0);
   TT_HIP_CHECK(hipGetLastError());
   return TENTACO_OK;
 }
@@ -1164,8 +1165,8 @@ int launch_batch_prep(const BatchLaunch& L, int N, hipStream_t st) {
   const int64_t work = std::max<int64_t>((f1 - f0) >> 2, b1 - b0);
   const int qshift = L.R == 16 ? 2 : L.R == 32 ? 3 : 4;  // log2(R / 4), R in {16, 32, 64}
   batch_prep_kernel<<<grid_for(work, 256, 16384), 256, 0, st>>>(
-      L.tasks, t_lo, t_hi, L.fill_off, f0 >> 2, f1 >> 2, L.bnd_off, b0, b1, L.counter + N, L.NB,
-      L.full_fill ? 1 : 0, L.R, qshift);
+      L.tasks, t_lo, t_hi, L.fill_off, f0 >> 2, f1 >> 2, L.bnd_off, b0, b1, L.counter + N, L.nb[N],
+      L.fam[N] != kEdgeParallel ? 1 : 0, L.R, qshift);
   TT_HIP_CHECK(hipGetLastError());
   return TENTACO_OK;
 }

@@ -228,3 +228,55 @@ def test_batch_rows_without_nonzeros_are_zero(gpu):
     bt.close()
     for H in tasks:
         H.close()
+
+
+def test_batch_per_order_tiles(gpu):
+    """A different tile and schedule per tensor order (edge-serial for order
+    3, edge-parallel with other blocks and groups for orders 4 and 5), eager and
+    replayed, against the oracle with poisoned outputs; configure() then sets
+    every order again, and autotune leaves a valid tile per order."""
+    R = 32
+    tensors = _tensors(22, n=6)
+    assert sorted({B.order for B in tensors}) == [3, 4, 5]
+    tasks, hosts, facs_h, facs_d, outs, rows = _setup(tensors, R, 23)
+    refs = [oracle.mttkrp(B, f, "taco", fp64=True) for B, f in zip(hosts, facs_h)]
+    bt = tt.Batch(tasks, R)
+    bt.set_operands(facs_d, outs, rows)
+    tiles = {3: (0, 256, 8, 1), 4: (1, 128, 32, 0), 5: (1, 512, 4, 1)}
+    for N, tile in tiles.items():
+        bt.configure_order(N, *tile)
+    for N, tile in tiles.items():
+        assert bt.config(N) == tile
+
+    def check(tag):
+        for t, (o, ref) in enumerate(zip(outs, refs)):
+            assert oracle.rel_l2(o.numpy(), ref) <= REL_L2_TOL, (tag, t)
+
+    for o in outs:
+        o.upload(np.full(o.shape, 9.0, np.float32))
+    bt.run()
+    tt.synchronize()
+    check("eager")
+    bt.capture()
+    for rep in range(2):
+        for o in outs:
+            o.upload(np.full(o.shape, -2.0 - rep, np.float32))
+        bt.launch()
+        tt.synchronize()
+        check(("graph", rep))
+    bt.configure(1, 256, 16)
+    bt.set_schedule(0)
+    for N in tiles:
+        assert bt.config(N) == (1, 256, 16, 0)
+    _, ms = bt.autotune(reps=1)
+    assert ms > 0
+    for N in tiles:
+        fam, blk, grp, sc = bt.config(N)
+        assert fam in (0, 1) and blk in (128, 256, 512) and grp in (4, 8, 16, 32, 64)
+        assert sc in (0, 1)
+    for o in outs:
+        o.upload(np.full(o.shape, 5.0, np.float32))
+    bt.run()
+    tt.synchronize()
+    check("tuned")
+    bt.close()
