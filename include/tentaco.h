@@ -207,7 +207,10 @@ int tentaco_csf_free(tentaco_csf* h);
 int tentaco_csf_info(const tentaco_csf* h, int* order, int64_t* nnz,
                      int32_t* nodes);
 /* Reserve the per-handle workspace a variant needs, so that later calls on
- * this handle allocate nothing (required before graph capture). For 3-mode
+ * this handle with at most dims[0] output rows allocate nothing (required
+ * before graph capture). Edge variants need the partition brackets; the
+ * edge-serial family also accumulates its rows in FP64 (output rows x R
+ * doubles), which keeps atomics on giant slices accurate. For 3-mode
  * handles it also builds the slice-row structures: the expansion (4 B per
  * nonzero plus 4 B per root node) and, for short fibres whose slices fit,
  * the padded slice layout (8 B per slot, at most 2 slots per nonzero). */

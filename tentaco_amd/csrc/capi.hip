@@ -325,7 +325,7 @@ int host_mttkrp(int order, taco_tensor_t* A, taco_tensor_t* B,
   const int v = resolve_variant(h, R, variant);
   const Variant vr = variant_of(v);
   if (vr.family < 0) return fail(TENTACO_EINVAL, "variant %d out of range", variant);
-  rc = reserve_ws(h, mttkrp_workspace_bytes(h->view(), R, vr));
+  rc = reserve_ws(h, mttkrp_workspace_bytes(h->view(), R, vr, a_rows));
   if (rc) return rc;
   if (vr.family == kSliceRow) {
     rc = ensure_expand3(h, nullptr);  // per-handle structure, outside the timed region
@@ -594,11 +594,11 @@ int tentaco_csf_reserve(tentaco_csf* h, int R, int variant) {
   if (variant == TENTACO_VARIANT_AUTO && h->tune.lookup(R) < 0) {
     // AUTO may pick any variant later: reserve for the largest need
     for (int v = 0; v < kNumVariants; ++v)
-      need = std::max(need, mttkrp_workspace_bytes(h->view(), R, variant_of(v)));
+      need = std::max(need, mttkrp_workspace_bytes(h->view(), R, variant_of(v), h->dims[0]));
   } else {
     const Variant vr = variant_of(resolve_variant(h, R, variant));
     if (vr.family < 0) return fail(TENTACO_EINVAL, "variant %d out of range", variant);
-    need = std::max(need, mttkrp_workspace_bytes(h->view(), R, vr));
+    need = std::max(need, mttkrp_workspace_bytes(h->view(), R, vr, h->dims[0]));
   }
   // the slice-row kernel's 3-mode expansion, so that no later call allocates
   int rc = ensure_expand3(h, nullptr);
@@ -693,7 +693,7 @@ int tentaco_mttkrp_dev(tentaco_csf* h, int R, const float* const* factors_dev, f
     if (rc) return rc;
   }
   const CsfDev B = h->view();
-  rc = reserve_ws(h, mttkrp_workspace_bytes(B, R, vr));
+  rc = reserve_ws(h, mttkrp_workspace_bytes(B, R, vr, a_rows));
   if (rc) return rc;
   const bool dense = h->dense_root && a_rows == h->nodes[0];
   return launch_mttkrp(B, F, R, A_dev, a_rows, dense, vr, h->ws,
@@ -711,7 +711,7 @@ static int sweep_impl(tentaco_csf* h, int R, const float* const* factors_dev, fl
   {
     size_t need = 0;
     for (int v = 0; v < kNumVariants; ++v)
-      need = std::max(need, mttkrp_workspace_bytes(h->view(), R, variant_of(v)));
+      need = std::max(need, mttkrp_workspace_bytes(h->view(), R, variant_of(v), a_rows));
     int rc = reserve_ws(h, need);
     if (rc) return rc;
   }

@@ -67,8 +67,12 @@ int default_variant();
 int lanes_per_row(int R);
 // Nonzeros one workgroup covers for an edge-family variant at rank R.
 int64_t nnz_per_block(const Variant& vr, int R);
-// Workspace bytes an MTTKRP call needs (0 for the slice-row family).
-size_t mttkrp_workspace_bytes(const CsfDev& B, int R, const Variant& vr);
+// Workspace bytes an MTTKRP call with a_rows output rows needs (0 for the
+// slice-row family): the
+// brackets (256-byte padded) for the edge families, then for the
+// edge-serial family the FP64 accumulation rows (a_rows x R doubles).
+size_t mttkrp_workspace_bytes(const CsfDev& B, int R, const Variant& vr, int32_t a_rows);
+size_t edge_brk_bytes(const CsfDev& B, int64_t NB);
 size_t ttv_workspace_bytes(const CsfDev& B);
 
 // Enqueue one MTTKRP on `stream`. A (a_rows x R) is fully overwritten;

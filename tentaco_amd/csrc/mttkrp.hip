@@ -103,11 +103,16 @@ __host__ __device__ inline size_t edge_stage_ints(int order, int32_t NB) {
   return n;
 }
 
+// With W non-NULL (edge-serial only) every row output goes to the FP64
+// buffer W instead of A: slices complete inside a group are stored, the
+// groups' first and last slices are added with FP64 atomics. A giant slice
+// then takes its many group partials without FP32 rounding at each add.
 template <int N, int R, bool EP>
 __device__ __forceinline__ void edge_block(const CsfDev& B, const FactorsDev& F,
                                            float* __restrict__ A, int Rr,
                                            const int32_t* __restrict__ brk, int32_t b,
-                                           int32_t G, int32_t NB, unsigned char* smem) {
+                                           int32_t G, int32_t NB, unsigned char* smem,
+                                           double* __restrict__ W = nullptr) {
   using RO = Row<R>;
   using V = typename RO::V;
   constexpr int NV = RO::NV;
@@ -246,10 +251,12 @@ __device__ __forceinline__ void edge_block(const CsfDev& B, const FactorsDev& F,
 #pragma unroll
                     for (int j = 0; j < NV; ++j) head_acc[j] = acc[0][j];
                   } else {
-                    RO::atomic_add(A, row, c, Rr, acc[0]);
+                    if (W) RO::atomic_add_f64(W, row, c, Rr, acc[0]);
+                    else RO::atomic_add(A, row, c, Rr, acc[0]);
                   }
                 } else {
-                  RO::store(A, row, c, Rr, acc[0]);
+                  if (!EP && W) RO::store_f64(W, row, c, Rr, acc[0]);
+                  else RO::store(A, row, c, Rr, acc[0]);
                 }
               }
               RO::zero(acc[k]);
@@ -296,7 +303,10 @@ __device__ __forceinline__ void edge_block(const CsfDev& B, const FactorsDev& F,
     // fold the open nodes into the open slice
 #pragma unroll
     for (int k = NL - 1; k >= 1; --k) RO::hfma(acc[k - 1], acc[k], rowc[k]);
-    if constexpr (!EP) RO::atomic_add(A, CRD(0, cur[0]), c, Rr, acc[0]);
+    if constexpr (!EP) {
+      if (W) RO::atomic_add_f64(W, CRD(0, cur[0]), c, Rr, acc[0]);
+      else RO::atomic_add(A, CRD(0, cur[0]), c, Rr, acc[0]);
+    }
   }
 
   if constexpr (EP) {
@@ -387,9 +397,24 @@ template <int N, int R, bool EP>
 __global__ __launch_bounds__(512) void edge_kernel(CsfDev B, FactorsDev F,
                                                    float* __restrict__ A, int Rr,
                                                    const int32_t* __restrict__ brk,
-                                                   int32_t G, int32_t NB) {
+                                                   int32_t G, int32_t NB,
+                                                   double* __restrict__ W) {
   extern __shared__ __align__(16) unsigned char smem[];
-  edge_block<N, R, EP>(B, F, A, Rr, brk, blockIdx.x, G, NB, smem);
+  edge_block<N, R, EP>(B, F, A, Rr, brk, blockIdx.x, G, NB, smem, W);
+}
+
+// A = FP32(W) over n elements (the edge-serial family's FP64 rows).
+__global__ __launch_bounds__(256) void narrow_kernel(const double* __restrict__ W,
+                                                     float* __restrict__ A, int64_t n) {
+  const int64_t nthr = int64_t(gridDim.x) * blockDim.x;
+  const int64_t n2 = n >> 1;
+  const double2* W2 = reinterpret_cast<const double2*>(W);
+  float2* A2 = reinterpret_cast<float2*>(A);
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n2; i += nthr) {
+    const double2 w = W2[i];
+    A2[i] = make_float2(float(w.x), float(w.y));
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0 && (n & 1)) A[n - 1] = float(W[n - 1]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1276,10 +1301,17 @@ int launch_typed(const CsfDev& B, const FactorsDev& F, int Rr, float* A,
   const int64_t NB = nnz_per_block(vr, Rr);
   const int64_t nblk = (int64_t(B.nnz) + NB - 1) / NB;
   int32_t* brk = static_cast<int32_t*>(ws);
+  // edge-serial: rows accumulate in FP64 (W, after the brackets), narrowed
+  // to A at the end; the prep zeroes W instead of A
+  double* W = (vr.family == kEdgeSerial && nblk > 0)
+                  ? reinterpret_cast<double*>(static_cast<char*>(ws) + edge_brk_bytes(B, NB))
+                  : nullptr;
   {
-    const int64_t work = std::max<int64_t>(a_elems / 4, nblk + 1);
+    float* Z = W ? reinterpret_cast<float*>(W) : A;
+    const int64_t z_elems = W ? 2 * a_elems : a_elems;
+    const int64_t work = std::max<int64_t>(z_elems / 4, nblk + 1);
     edge_prep_kernel<<<grid_for(work, 256, 16384), 256, 0, st>>>(
-        B, A, a_elems, brk, int32_t(nblk), int32_t(NB));
+        B, Z, z_elems, brk, int32_t(nblk), int32_t(NB));
     TT_HIP_CHECK(hipGetLastError());
   }
   if (nblk == 0) return TENTACO_OK;
@@ -1291,7 +1323,12 @@ int launch_typed(const CsfDev& B, const FactorsDev& F, int Rr, float* A,
                   (long long)NB, stage);
     ktimer_begin(st);
     edge_kernel<N, R, false><<<int(nblk), vr.block, stage, st>>>(
-        B, F, A, Rr, brk, vr.group, int32_t(NB));
+        B, F, A, Rr, brk, vr.group, int32_t(NB), W);
+    TT_HIP_CHECK(hipGetLastError());
+    ktimer_end(st, "edge_kernel");
+    narrow_kernel<<<grid_for(a_elems / 2 + 1, 256, 16384), 256, 0, st>>>(W, A, a_elems);
+    TT_HIP_CHECK(hipGetLastError());
+    return TENTACO_OK;
   } else {
     const size_t lds = edge_ep_bytes(vr.block, lpr, Rr) + stage;
     if (lds > 65536)
@@ -1299,7 +1336,7 @@ int launch_typed(const CsfDev& B, const FactorsDev& F, int Rr, float* A,
                   vr.block, Rr, lds);
     ktimer_begin(st);
     edge_kernel<N, R, true><<<int(nblk), vr.block, lds, st>>>(
-        B, F, A, Rr, brk, vr.group, int32_t(NB));
+        B, F, A, Rr, brk, vr.group, int32_t(NB), nullptr);
   }
   TT_HIP_CHECK(hipGetLastError());
   ktimer_end(st, "edge_kernel");
@@ -1477,11 +1514,17 @@ bool variant_fits(int order, int R, int v) {
   return edge_ep_bytes(vr.block, lanes_per_row(R), R) + stage <= 65536;
 }
 
-size_t mttkrp_workspace_bytes(const CsfDev& B, int R, const Variant& vr) {
-  if (vr.family == kSliceRow || vr.family < 0) return 0;
-  const int64_t NB = nnz_per_block(vr, R);
+size_t edge_brk_bytes(const CsfDev& B, int64_t NB) {
   const int64_t nblk = (int64_t(B.nnz) + NB - 1) / NB;
-  return size_t(nblk + 1) * size_t(B.order - 1) * sizeof(int32_t);
+  const size_t bytes = size_t(nblk + 1) * size_t(B.order - 1) * sizeof(int32_t);
+  return (bytes + 255) / 256 * 256;
+}
+
+size_t mttkrp_workspace_bytes(const CsfDev& B, int R, const Variant& vr, int32_t a_rows) {
+  if (vr.family == kSliceRow || vr.family < 0) return 0;
+  const size_t brk = edge_brk_bytes(B, nnz_per_block(vr, R));
+  if (vr.family != kEdgeSerial) return brk;
+  return brk + size_t(std::max<int32_t>(a_rows, 0)) * size_t(R) * sizeof(double);
 }
 
 int launch_mttkrp_level(const CsfDev& B, const FactorsDev& F, int level, int R, float* A,

@@ -175,6 +175,50 @@ struct Row {
     }
   }
 
+  // FP64 row output (the edge-serial family's accumulation buffer): store,
+  // and device-scope FP64 atomic add (global_atomic_add_f64).
+  __device__ __forceinline__ static void store_f64(double* __restrict__ W, int64_t row, int c,
+                                                   int Rr, const V (&a)[NV]) {
+    if constexpr (R == 0) {
+      double* p = W + row * Rr;
+#pragma unroll
+      for (int j = 0; j < NV; ++j) {
+        const int k = col(j, c);
+        if (k < Rr) p[k] = double(a[j]);
+      }
+    } else {
+      double* p = W + row * R;
+#pragma unroll
+      for (int j = 0; j < NV; ++j) {
+        double2* q = reinterpret_cast<double2*>(p + col(j, c));
+        q[0] = make_double2(double(a[j].x), double(a[j].y));
+        q[1] = make_double2(double(a[j].z), double(a[j].w));
+      }
+    }
+  }
+
+  __device__ __forceinline__ static void atomic_add_f64(double* __restrict__ W, int64_t row, int c,
+                                                        int Rr, const V (&a)[NV]) {
+    if constexpr (R == 0) {
+      double* p = W + row * Rr;
+#pragma unroll
+      for (int j = 0; j < NV; ++j) {
+        const int k = col(j, c);
+        if (k < Rr) unsafeAtomicAdd(p + k, double(a[j]));
+      }
+    } else {
+      double* p = W + row * R;
+#pragma unroll
+      for (int j = 0; j < NV; ++j) {
+        double* q = p + col(j, c);
+        unsafeAtomicAdd(q + 0, double(a[j].x));
+        unsafeAtomicAdd(q + 1, double(a[j].y));
+        unsafeAtomicAdd(q + 2, double(a[j].z));
+        unsafeAtomicAdd(q + 3, double(a[j].w));
+      }
+    }
+  }
+
   // Sum a over the row groups of a wavefront (xor butterfly over the lane
   // bits above log2(LPR)); every lane ends with the wave total of its slots.
   __device__ __forceinline__ static void wave_sum(V (&a)[NV]) {
