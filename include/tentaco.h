@@ -411,9 +411,11 @@ int tentaco_batch_create(tentaco_csf* const* tasks, int ntasks, int R,
  * stream and synchronizes the device before returning. */
 int tentaco_batch_set_operands(tentaco_batch* b, const float* const* factors_dev,
                                float* const* outs_dev, const int32_t* out_rows);
-/* Tile configuration of every order group: family 0 edge-serial / 1
- * edge-parallel, block in {128, 256, 512}, group = nonzeros per lane group.
- * Drops a captured graph. */
+/* Tile configuration of every order group: family 1 (edge-parallel; the
+ * batch runs that walk only, and family 0 returns TENTACO_EUNSUPPORTED,
+ * since the edge-serial walk's per-group FP32 atomics lose accuracy on giant
+ * slices), block in {128, 256, 512}, group = nonzeros per lane group. Drops
+ * a captured graph. */
 int tentaco_batch_configure(tentaco_batch* b, int family, int block, int group);
 /* The tile and schedule (see tentaco_batch_set_schedule) of the tasks of one
  * tensor order (2..5) only; each order group runs its own persistent launch,
@@ -433,8 +435,8 @@ int tentaco_batch_launch(tentaco_batch* b, void* stream);
  * order group with the most items, in the current configuration. */
 int tentaco_batch_items(const tentaco_batch* b, int32_t* items, int32_t* grid,
                         int32_t* nb);
-/* Time every tile configuration for all orders at once (2 schedules x 2
- * families x 3 blocks x 5 groups) and keep the fastest, returned in
+/* Time every tile configuration for all orders at once (2 schedules x 3
+ * blocks x 5 groups) and keep the fastest, returned in
  * best_family / best_block / best_group. Then time every tile and schedule of
  * each order group in turn with the other groups held, keeping what is
  * faster (tentaco_batch_get_config reads the result); best_ms is the final

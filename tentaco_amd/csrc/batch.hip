@@ -66,9 +66,15 @@ void free_tables(tentaco_batch* b) {
   b->brk_bytes = 0;
 }
 
+// The batch runs the edge-parallel walk only: its block-level combine keeps
+// the FP32 atomics per output row few. The edge-serial walk adds one atomic
+// per lane group, and on giant slices that exceeded 1e-5 relative L2.
 int check_tile(int family, int block, int group) {
-  if (family != kEdgeSerial && family != kEdgeParallel)
-    return fail(TENTACO_EINVAL, "batch family must be 0 (edge-serial) or 1 (edge-parallel)");
+  if (family == kEdgeSerial)
+    return fail(TENTACO_EUNSUPPORTED,
+                "the batch runs the edge-parallel walk only (family 1); edge-serial tiles are "
+                "per-tensor (tentaco_mttkrp_dev), where rows accumulate in FP64");
+  if (family != kEdgeParallel) return fail(TENTACO_EINVAL, "batch family must be 1 (edge-parallel)");
   if (block != 128 && block != 256 && block != 512)
     return fail(TENTACO_EINVAL, "batch block %d not in {128,256,512}", block);
   if (group < 1 || group > 1024) return fail(TENTACO_EINVAL, "batch group %d", group);
@@ -393,7 +399,7 @@ int tentaco_batch_autotune(tentaco_batch* b, int reps, void* stream, int* best_f
   float bm = -1.f;
   int bf = -1, bb = 0, bg = 0, bs = 0;
   for (int sched = 0; sched < 2; ++sched)
-    for (int fam = 0; fam < 2; ++fam)
+    for (int fam = kEdgeParallel; fam <= kEdgeParallel; ++fam)
       for (int blk : {128, 256, 512})
         for (int grp : {4, 8, 16, 32, 64}) {
           float med = 0.f;
@@ -424,7 +430,7 @@ int tentaco_batch_autotune(tentaco_batch* b, int reps, void* stream, int* best_f
     int of = b->L.fam[N], ob = b->L.blk[N], og = b->L.grp[N], os = b->L.sch[N];
     const int f0 = of, b0 = ob, g0 = og, s0 = os;
     for (int sched = 0; sched < 2; ++sched)
-    for (int fam = 0; fam < 2; ++fam)
+    for (int fam = kEdgeParallel; fam <= kEdgeParallel; ++fam)
       for (int blk : {128, 256, 512})
         for (int grp : {4, 8, 16, 32, 64}) {
           if (fam == f0 && blk == b0 && grp == g0 && sched == s0) continue;

@@ -163,12 +163,11 @@ struct BatchLaunch {
   const int64_t* item_off = nullptr;    // device, prefix of items per task (ntasks + 1)
   int32_t total_items = 0;
   int32_t* counter = nullptr;           // device work counter
-  // Tile per order (index = order): family, block, group and nonzeros per
-  // item (nb = block / lanes-per-row * group). For the edge-parallel family
-  // the prep skips the output rows that root nodes with nonzeros store (their
+  // Tile per order (index = order): family (edge-parallel only), block,
+  // group and nonzeros per item (nb = block / lanes-per-row * group). The
+  // prep skips the output rows that root nodes with nonzeros store (their
   // bits in the tasks' rowbits), except the rows of the slices holding an
-  // item's first or last nonzero, which take atomics; the edge-serial family
-  // zero-fills whole outputs.
+  // item's first or last nonzero, which take atomics.
   // sch[N] is how the order's persistent workgroups take items: 0 from a
   // device work counter, 1 static cyclic.
   int fam[kMaxOrder + 1] = {}, blk[kMaxOrder + 1] = {}, grp[kMaxOrder + 1] = {};

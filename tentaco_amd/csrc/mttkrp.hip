@@ -999,12 +999,12 @@ __device__ __forceinline__ void zero_row(float* A, int64_t row, int R) {
 
 // Prep of the order group holding sorted tasks [t_lo, t_hi): reset that
 // order's work counter, write its brackets (entries [b_lo, b_hi)) and zero
-// the outputs (quads [q_lo, q_hi)). With full_fill every row is zeroed.
-// Otherwise (edge-parallel) a row whose bit is set in the task's rowbits is
-// skipped, because a root node with nonzeros stores it, unless it is the row
-// of a slice holding position b*NB or b*NB - 1 of some boundary b: those are
-// the walk's only atomically updated rows (the slices holding an item's first
-// and last nonzero) and are zeroed with the brackets.
+// the outputs (quads [q_lo, q_hi)). A row whose bit is set in the task's
+// rowbits is skipped, because a root node with nonzeros stores it, unless it
+// is the row of a slice holding position b*NB or b*NB - 1 of some boundary
+// b: those are the edge-parallel walk's only atomically updated rows (the
+// slices holding an item's first and last nonzero) and are zeroed with the
+// brackets.
 __global__ __launch_bounds__(256) void batch_prep_kernel(const BatchTaskDev* __restrict__ tasks,
                                                          int32_t t_lo, int32_t t_hi,
                                                          const int64_t* __restrict__ fill_off,
@@ -1012,8 +1012,7 @@ __global__ __launch_bounds__(256) void batch_prep_kernel(const BatchTaskDev* __r
                                                          const int64_t* __restrict__ bnd_off,
                                                          int64_t b_lo, int64_t b_hi,
                                                          int32_t* __restrict__ counter,
-                                                         int32_t NB, int full_fill, int R,
-                                                         int qshift) {
+                                                         int32_t NB, int R, int qshift) {
   const int64_t tid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t nthr = int64_t(gridDim.x) * blockDim.x;
   if (tid == 0) *counter = 0;
@@ -1064,7 +1063,7 @@ __global__ __launch_bounds__(256) void batch_prep_kernel(const BatchTaskDev* __r
       }
       uint32_t w[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) w[u] = (dst[u] && !full_fill) ? *wp[u] : 0u;
+      for (int u = 0; u < U; ++u) w[u] = dst[u] ? *wp[u] : 0u;
 #pragma unroll
       for (int u = 0; u < U; ++u)
         if (dst[u] && !((w[u] >> sh[u]) & 1u)) *dst[u] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1084,7 +1083,7 @@ __global__ __launch_bounds__(256) void batch_prep_kernel(const BatchTaskDev* __r
       T.brk[bi * (N - 1) + m] = x;
       key = x;
     }
-    if (!full_fill) {
+    {
       zero_row(T.A, T.B.crd0[key], R);
       // position p - 1 (the previous item's last nonzero) lies in another
       // slice only when slice `key` starts exactly at p; its first nonzero
@@ -1163,19 +1162,13 @@ int launch_batch_order(const BatchLaunch& L, hipStream_t st) {
   const int32_t i_lo = L.grp_item[N], i_hi = L.grp_item[N + 1];
   if (i_hi <= i_lo) return TENTACO_OK;
   const int lpr = Row<R>::LPR;
-  const int fam = L.fam[N], blk = L.blk[N];
-  size_t lds = edge_stage_ints(N, L.nb[N]) * sizeof(int32_t);
-  if (fam == kEdgeParallel) lds += edge_ep_bytes(blk, lpr, L.R);
+  const int blk = L.blk[N];
+  const size_t lds = edge_stage_ints(N, L.nb[N]) * sizeof(int32_t) + edge_ep_bytes(blk, lpr, L.R);
   if (lds > 65536) return fail(TENTACO_EUNSUPPORTED, "batch LDS %zu B", lds);
   const int grid = std::max(1, std::min<int>(L.grp_grid[N], i_hi - i_lo));
-  if (fam == kEdgeParallel)
-    batch_kernel<N, R, true><<<grid, blk, lds, st>>>(
-        L.tasks, L.grp_task[N], L.grp_task[N + 1], L.item_off, i_lo, i_hi, L.counter + N, L.R,
-        L.grp[N], L.nb[N], L.sch[N] == 0);
-  else
-    batch_kernel<N, R, false><<<grid, blk, lds, st>>>(
-        L.tasks, L.grp_task[N], L.grp_task[N + 1], L.item_off, i_lo, i_hi, L.counter + N, L.R,
-        L.grp[N], L.nb[N], L.sch[N] == 0);
+  batch_kernel<N, R, true><<<grid, blk, lds, st>>>(
+      L.tasks, L.grp_task[N], L.grp_task[N + 1], L.item_off, i_lo, i_hi, L.counter + N, L.R,
+      L.grp[N], L.nb[N], L.sch[N] == 0);
   TT_HIP_CHECK(hipGetLastError());
   return TENTACO_OK;
 }
@@ -1191,7 +1184,7 @@ int launch_batch_prep(const BatchLaunch& L, int N, hipStream_t st) {
   const int qshift = L.R == 16 ? 2 : L.R == 32 ? 3 : 4;  // log2(R / 4), R in {16, 32, 64}
   batch_prep_kernel<<<grid_for(work, 256, 16384), 256, 0, st>>>(
       L.tasks, t_lo, t_hi, L.fill_off, f0 >> 2, f1 >> 2, L.bnd_off, b0, b1, L.counter + N, L.nb[N],
-      L.fam[N] != kEdgeParallel ? 1 : 0, L.R, qshift);
+      L.R, qshift);
   TT_HIP_CHECK(hipGetLastError());
   return TENTACO_OK;
 }
@@ -1235,14 +1228,11 @@ int launch_batch_typed(const BatchLaunch& L, hipStream_t st) {
 template <int N, int R>
 int batch_occupancy_order(int family, int block, int Rr, int32_t NB, int* per_cu) {
   const int lpr = Row<R>::LPR;
-  hipError_t e;
-  size_t lds = edge_stage_ints(N, NB) * sizeof(int32_t);
-  if (family == kEdgeParallel) {
-    lds += edge_ep_bytes(block, lpr, Rr);
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, batch_kernel<N, R, true>, block, lds);
-  } else {
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, batch_kernel<N, R, false>, block, lds);
-  }
+  if (family != kEdgeParallel)
+    return fail(TENTACO_EUNSUPPORTED, "the batch runs the edge-parallel walk only");
+  const size_t lds = edge_stage_ints(N, NB) * sizeof(int32_t) + edge_ep_bytes(block, lpr, Rr);
+  const hipError_t e =
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, batch_kernel<N, R, true>, block, lds);
   if (e != hipSuccess) return fail(TENTACO_EHIP, "occupancy query: %s", hipGetErrorString(e));
   return TENTACO_OK;
 }

@@ -48,8 +48,9 @@ def test_batch_matches_oracle_every_config(gpu):
     refs = [oracle.mttkrp(B, f, "taco", fp64=True) for B, f in zip(hosts, facs_h)]
     bt = tt.Batch(tasks, R)
     bt.set_operands(facs_d, outs, rows)
-    configs = [(sc, f, b, g) for sc in (0, 1) for f in (0, 1)
-               for b in (128, 256, 512) for g in (4, 16, 64)]
+    with pytest.raises(tt.TentacoError):
+        bt.configure(0, 256, 16)  # edge-serial tiles are per-tensor only
+    configs = [(sc, 1, b, g) for sc in (0, 1) for b in (128, 256, 512) for g in (4, 16, 64)]
     for sched, fam, blk, grp in configs:
         bt.configure(fam, blk, grp)
         bt.set_schedule(sched)
@@ -94,7 +95,7 @@ def test_batch_autotune_picks_a_valid_config(gpu):
     bt = tt.Batch(tasks, R)
     bt.set_operands(facs_d, outs, rows)
     (fam, blk, grp), ms = bt.autotune(reps=2)
-    assert fam in (0, 1) and blk in (128, 256, 512) and grp in (4, 8, 16, 32, 64) and ms > 0
+    assert fam == 1 and blk in (128, 256, 512) and grp in (4, 8, 16, 32, 64) and ms > 0
     bt.run()
     for B, f, o in zip(hosts, facs_h, outs):
         assert oracle.rel_l2(o.numpy(), oracle.mttkrp(B, f, "taco", fp64=True)) <= REL_L2_TOL
@@ -110,7 +111,7 @@ def test_batch_other_ranks_eager_and_graph(gpu, R):
     refs = [oracle.mttkrp(B, f, "taco", fp64=True) for B, f in zip(hosts, facs_h)]
     bt = tt.Batch(tasks, R)
     bt.set_operands(facs_d, outs, rows)
-    for fam, blk, grp in ((0, 256, 8), (1, 128, 64)):
+    for fam, blk, grp in ((1, 256, 8), (1, 128, 64)):
         bt.configure(fam, blk, grp)
         for o in outs:
             o.upload(np.full(o.shape, 9.0, np.float32))
@@ -154,9 +155,9 @@ def test_batch_rows_without_nonzeros_are_zero(gpu):
     """Outputs whose rows have no nonzeros: a compressed root with gaps, root
     nodes whose only children are empty fibres, output rows beyond the
     largest coordinate, and one Zipf tensor whose slices cross many items.
-    Such rows must read exactly zero after every run (the edge-parallel
-    family writes them from its zero items, the edge-serial family from the
-    prep's fill), with outputs poisoned before each run, eager and replayed."""
+    Such rows must read exactly zero after every run (the prep zeroes the
+    rows whose bitmap bit is clear), with outputs poisoned before each run,
+    eager and replayed."""
     from helpers import random_coo
     rng = np.random.default_rng(31)
     R = 32
@@ -208,7 +209,7 @@ def test_batch_rows_without_nonzeros_are_zero(gpu):
             assert oracle.rel_l2(got, ref) <= REL_L2_TOL, (tag, t)
 
     for sched in (0, 1):
-        for fam, blk, grp in ((1, 128, 64), (1, 256, 4), (1, 512, 16), (0, 256, 8)):
+        for fam, blk, grp in ((1, 128, 64), (1, 256, 4), (1, 512, 16), (1, 128, 8)):
             bt.configure(fam, blk, grp)
             bt.set_schedule(sched)
             for o in outs:
@@ -231,9 +232,9 @@ def test_batch_rows_without_nonzeros_are_zero(gpu):
 
 
 def test_batch_per_order_tiles(gpu):
-    """A different tile and schedule per tensor order (edge-serial for order
-    3, edge-parallel with other blocks and groups for orders 4 and 5), eager and
-    replayed, against the oracle with poisoned outputs; configure() then sets
+    """A different tile and schedule per tensor order (blocks 256 / 128 /
+    512, groups 8 / 32 / 4, static / counter / static for orders 3 / 4 / 5),
+    eager and replayed, against the oracle with poisoned outputs; configure() then sets
     every order again, and autotune leaves a valid tile per order."""
     R = 32
     tensors = _tensors(22, n=6)
@@ -242,7 +243,9 @@ def test_batch_per_order_tiles(gpu):
     refs = [oracle.mttkrp(B, f, "taco", fp64=True) for B, f in zip(hosts, facs_h)]
     bt = tt.Batch(tasks, R)
     bt.set_operands(facs_d, outs, rows)
-    tiles = {3: (0, 256, 8, 1), 4: (1, 128, 32, 0), 5: (1, 512, 4, 1)}
+    tiles = {3: (1, 256, 8, 1), 4: (1, 128, 32, 0), 5: (1, 512, 4, 1)}
+    with pytest.raises(tt.TentacoError):
+        bt.configure_order(3, 0, 256, 8, 0)  # edge-serial tiles are per-tensor only
     for N, tile in tiles.items():
         bt.configure_order(N, *tile)
     for N, tile in tiles.items():
@@ -272,7 +275,7 @@ def test_batch_per_order_tiles(gpu):
     assert ms > 0
     for N in tiles:
         fam, blk, grp, sc = bt.config(N)
-        assert fam in (0, 1) and blk in (128, 256, 512) and grp in (4, 8, 16, 32, 64)
+        assert fam == 1 and blk in (128, 256, 512) and grp in (4, 8, 16, 32, 64)
         assert sc in (0, 1)
     for o in outs:
         o.upload(np.full(o.shape, 5.0, np.float32))
