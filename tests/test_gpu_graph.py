@@ -1,0 +1,120 @@
+"""Per-tensor calls captured by the caller into a hipGraph (the documented
+use: reserve the handle's workspace first, then capture on a non-default
+stream): MTTKRP with the edge-serial family (FP64 rows: prep, walk and
+narrowing kernels), the edge-parallel family and the padded slice-row
+kernel, and a TTV after its plan is built. Each graph is replayed several
+times with eager work and poisoned outputs in between, and every replay is
+checked against the eager result and the oracle."""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle
+import tentaco_amd as tt
+from helpers import REL_L2_TOL, factors_for, skewed_coo
+from tentaco_amd import metrics
+
+pytestmark = pytest.mark.gpu
+
+CAPTURE_THREAD_LOCAL = 1
+
+
+def _hip():
+    for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+        try:
+            h = ctypes.CDLL(name)
+            break
+        except OSError:
+            h = None
+    assert h is not None
+    vp = ctypes.c_void_p
+    h.hipStreamBeginCapture.argtypes = [vp, ctypes.c_int]
+    h.hipStreamEndCapture.argtypes = [vp, ctypes.POINTER(vp)]
+    h.hipGraphInstantiate.argtypes = [ctypes.POINTER(vp), vp, vp, ctypes.c_char_p, ctypes.c_size_t]
+    h.hipGraphLaunch.argtypes = [vp, vp]
+    h.hipGraphExecDestroy.argtypes = [vp]
+    h.hipGraphDestroy.argtypes = [vp]
+    return h
+
+
+def _capture(hip, stream, record):
+    g, ex = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipStreamBeginCapture(stream.h, CAPTURE_THREAD_LOCAL) == 0
+    try:
+        record()
+    finally:
+        assert hip.hipStreamEndCapture(stream.h, ctypes.byref(g)) == 0
+    assert hip.hipGraphInstantiate(ctypes.byref(ex), g, None, None, 0) == 0
+    return g, ex
+
+
+def test_mttkrp_variants_replay_in_a_graph(gpu):
+    hip = _hip()
+    rng = np.random.default_rng(91)
+    R = 32
+    coords, vals = skewed_coo(rng, (300, 400, 500), 150_000)
+    skewed = tt.Csf.from_coo((300, 400, 500), coords, vals)
+    uniform = metrics.workload_mttkrp3_uniform(nnz=80_000, dim=1024, seed=92)  # padded layout
+    names = tt.variants()
+    s = tt.Stream()
+    other = tt.DeviceArray.empty((1 << 18,))
+    cases = [(skewed, tt.default_variant()), (skewed, names.index("ep_b256_g16")),
+             (uniform, names.index("sr_b256"))]
+    for B, v in cases:
+        facs = factors_for(rng, B, R)
+        ref = oracle.mttkrp(B, facs, "taco", fp64=True)
+        H = tt.DeviceCsf(B)
+        fd = [tt.DeviceArray.from_numpy(f) for f in facs]
+        A = tt.DeviceArray.empty((B.dims[0], R))
+        H.reserve(R, v)
+        H.mttkrp(fd, A, R, v, stream=s.h)
+        s.synchronize()
+        eager = A.numpy()
+        assert oracle.rel_l2(eager, ref) <= REL_L2_TOL, names[v]
+        g, ex = _capture(hip, s, lambda: H.mttkrp(fd, A, R, v, stream=s.h))
+        try:
+            for rep in range(3):
+                A.upload(np.full((B.dims[0], R), 7.0 + rep, np.float32))
+                other.upload(np.full((1 << 18,), float(rep), np.float32))  # eager work between
+                assert hip.hipGraphLaunch(ex, s.h) == 0
+                s.synchronize()
+                got = A.numpy()
+                assert oracle.rel_l2(got, eager) <= 1e-6, (names[v], rep)
+                assert oracle.rel_l2(got, ref) <= REL_L2_TOL, (names[v], rep)
+                if names[v].startswith("sr_"):
+                    assert np.array_equal(got, eager), rep  # no atomics: bitwise
+        finally:
+            hip.hipGraphExecDestroy(ex)
+            hip.hipGraphDestroy(g)
+        H.close()
+    s.close()
+
+
+def test_ttv_replays_in_a_graph(gpu):
+    hip = _hip()
+    B = metrics.workload_zipf(3, 20_000, 2_000_000, seed=93)
+    c = np.random.default_rng(94).uniform(-1, 1, 20_000).astype(np.float32)
+    ref = oracle.ttv(B, c, "taco", fp64=True)
+    H = tt.DeviceCsf(B)
+    cd = tt.DeviceArray.from_numpy(c)
+    y = tt.DeviceArray.empty((B.nodes(1),))
+    s = tt.Stream()
+    H.ttv(cd, y, stream=s.h)  # builds the plan outside the capture
+    s.synchronize()
+    eager = y.numpy()
+    g, ex = _capture(hip, s, lambda: H.ttv(cd, y, stream=s.h))
+    try:
+        for rep in range(3):
+            y.upload(np.full(B.nodes(1), -3.0 - rep, np.float32))
+            assert hip.hipGraphLaunch(ex, s.h) == 0
+            s.synchronize()
+            got = y.numpy()
+            assert np.array_equal(got, eager), rep  # the unit kernel is deterministic
+            assert oracle.rel_l2(got, ref) <= REL_L2_TOL, rep
+    finally:
+        hip.hipGraphExecDestroy(ex)
+        hip.hipGraphDestroy(g)
+    H.close()
+    s.close()
