@@ -255,7 +255,7 @@ int resolve_variant(const tentaco_csf* h, int R, int variant) {
   if (variant == TENTACO_VARIANT_AUTO) {
     const int tuned = h->tune.lookup(R);
     if (tuned >= 0) return tuned;
-    return auto_variant(h->stats, R);
+    return auto_variant(h->stats, h->order, R);
   }
   return variant;
 }
@@ -622,7 +622,7 @@ static int auto_with_selector(tentaco_csf* h, int R, const float* const* factors
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   TT_HIP_CHECK(hipStreamIsCapturing(st, &cs));
   if (cs != hipStreamCaptureStatusNone) {
-    *out = auto_variant(h->stats, R);
+    *out = auto_variant(h->stats, h->order, R);
     return TENTACO_OK;
   }
   float f[TENTACO_NUM_FEATURES];

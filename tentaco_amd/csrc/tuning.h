@@ -50,7 +50,7 @@ struct CsfStats {
 };
 
 // Heuristic choice when the handle has not been autotuned: slice-row when the
-// root slices are many and even, edge-serial otherwise.
-int auto_variant(const CsfStats& s, int R);
+// root slices are many and even, edge-parallel otherwise.
+int auto_variant(const CsfStats& s, int order, int R);
 
 }  // namespace tt
