@@ -496,9 +496,9 @@ int tentaco_timer_stop(void* stream, float* ms);
 
 /* Kernel timer: while on, every call records a hipEvent pair on its launch
  * stream around its main kernel only (slice_kernel, slice_short_kernel,
- * slice_pad_kernel,
- * edge_kernel, level_kernel, ttv_kernel, batch_kernel or copy_kernel; not the
- * partition/zero pre-pass or the edge-serial FP64-to-FP32 pass), at most 4096
+ * slice_pad_kernel, edge_kernel, level_kernel, ttv_unit_kernel, ttv_kernel,
+ * batch_kernel or copy_kernel; not the partition/zero pre-pass or the
+ * edge-serial FP64-to-FP32 pass), at most 4096
  * launches; never inside a stream capture. stop() turns it off and returns
  * the summed duration, the launch count and the last kernel's name. No
  * reference counterpart (GPUTimer, gpu_library.h:949-988, brackets the whole
