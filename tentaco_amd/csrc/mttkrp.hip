@@ -103,10 +103,12 @@ __host__ __device__ inline size_t edge_stage_ints(int order, int32_t NB) {
   return n;
 }
 
-// With W non-NULL (edge-serial only) every row output goes to the FP64
-// buffer W instead of A: slices complete inside a group are stored, the
-// groups' first and last slices are added with FP64 atomics. A giant slice
-// then takes its many group partials without FP32 rounding at each add.
+// The edge-serial family (EP false) needs W: every row output goes to the
+// FP64 buffer W instead of A. Slices complete inside a group are stored; the
+// groups' first and last slices are combined across the wave's groups by a
+// segmented suffix scan over shuffles, and each slice's sum per wave is
+// added with one FP64 atomic. A giant slice then takes one atomic per wave,
+// without FP32 rounding at each add.
 template <int N, int R, bool EP>
 __device__ __forceinline__ void edge_block(const CsfDev& B, const FactorsDev& F,
                                            float* __restrict__ A, int Rr,
@@ -247,16 +249,11 @@ __device__ __forceinline__ void edge_block(const CsfDev& B, const FactorsDev& F,
                 const int64_t row = CRD(0, old);
                 if (old == head && !head_closed) {
                   head_closed = true;
-                  if constexpr (EP) {
 #pragma unroll
-                    for (int j = 0; j < NV; ++j) head_acc[j] = acc[0][j];
-                  } else {
-                    if (W) RO::atomic_add_f64(W, row, c, Rr, acc[0]);
-                    else RO::atomic_add(A, row, c, Rr, acc[0]);
-                  }
+                  for (int j = 0; j < NV; ++j) head_acc[j] = acc[0][j];
                 } else {
-                  if (!EP && W) RO::store_f64(W, row, c, Rr, acc[0]);
-                  else RO::store(A, row, c, Rr, acc[0]);
+                  if constexpr (EP) RO::store(A, row, c, Rr, acc[0]);
+                  else RO::store_f64(W, row, c, Rr, acc[0]);
                 }
               }
               RO::zero(acc[k]);
@@ -303,10 +300,83 @@ __device__ __forceinline__ void edge_block(const CsfDev& B, const FactorsDev& F,
     // fold the open nodes into the open slice
 #pragma unroll
     for (int k = NL - 1; k >= 1; --k) RO::hfma(acc[k - 1], acc[k], rowc[k]);
-    if constexpr (!EP) {
-      if (W) RO::atomic_add_f64(W, CRD(0, cur[0]), c, Rr, acc[0]);
-      else RO::atomic_add(A, CRD(0, cur[0]), c, Rr, acc[0]);
+  }
+
+  if constexpr (!EP) {
+    // Entries e = 2 * gw + q of the wave's NGW groups: q = 0 is the group's
+    // first slice (its partial if it closed inside the run, else 0), q = 1
+    // the slice open at the run's end. Keys do not decrease with e; idle
+    // groups (trailing) hold INT_MAX. Suffix scan: at stride s, entry e adds
+    // entry e + s when both have the same key, so each key's first entry ends
+    // with the key's sum over the wave. Stride 1 pairs (gw, 0) with (gw, 1)
+    // and (gw, 1) with (gw + 1, 0); an even stride pairs (gw, q) with
+    // (gw + s / 2, q). Every lane runs every shuffle.
+    constexpr int NGW = 64 / LPR;
+    const int lane = int(threadIdx.x) & 63;
+    const int gw = lane / LPR;
+    const int32_t k0 = active ? head : INT_MAX;
+    const int32_t k1 = active ? cur[0] : INT_MAX;
+    V v0[NV], v1[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      v0[j] = head_acc[j];
+      v1[j] = acc[0][j];
     }
+    auto shfl_v = [&](const V (&x)[NV], int src, V (&out)[NV]) {
+#pragma unroll
+      for (int j = 0; j < NV; ++j) {
+        if constexpr (R == 0) {
+          out[j] = __shfl(x[j], src);
+        } else {
+          out[j].x = __shfl(x[j].x, src);
+          out[j].y = __shfl(x[j].y, src);
+          out[j].z = __shfl(x[j].z, src);
+          out[j].w = __shfl(x[j].w, src);
+        }
+      }
+    };
+#pragma unroll
+    for (int st = 1; st < 2 * NGW; st <<= 1) {
+      const int dg = st == 1 ? 1 : st / 2;  // groups ahead of the source
+      const int src = (lane + dg * LPR) & 63;
+      const bool in_wave = gw + dg < NGW;
+      V a0[NV], a1[NV], b0[NV], b1[NV];
+      shfl_v(v0, src, b0);
+      shfl_v(v1, src, b1);
+      const int32_t n0 = __shfl(k0, src);
+      const int32_t n1 = __shfl(k1, src);
+      int32_t t0, t1;
+      bool ok0, ok1;
+      if (st == 1) {
+        // (gw, 0) + (gw, 1) in this lane; (gw, 1) + (gw + 1, 0)
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+          a0[j] = v1[j];
+          a1[j] = b0[j];
+        }
+        t0 = k1;
+        t1 = n0;
+        ok0 = true;
+        ok1 = in_wave;
+      } else {
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+          a0[j] = b0[j];
+          a1[j] = b1[j];
+        }
+        t0 = n0;
+        t1 = n1;
+        ok0 = in_wave;
+        ok1 = in_wave;
+      }
+      if (ok0 && k0 != INT_MAX && t0 == k0) RO::add(v0, a0);
+      if (ok1 && k1 != INT_MAX && t1 == k1) RO::add(v1, a1);
+    }
+    // a key's first entry in the wave adds the sum
+    const int32_t kprev = __shfl(k1, (lane - LPR) & 63);  // (gw - 1, 1)
+    if (k0 != INT_MAX && (gw == 0 || kprev != k0))
+      RO::atomic_add_f64(W, CRD(0, k0), c, Rr, v0);
+    if (k1 != INT_MAX && k1 != k0) RO::atomic_add_f64(W, CRD(0, k1), c, Rr, v1);
   }
 
   if constexpr (EP) {
